@@ -1,0 +1,232 @@
+#!/usr/bin/env python3
+"""Headline benchmark: Ac2Standard propagation, FP32, FD order 8, 1000^3, one MI355X.
+
+One step is one time step of hpcscan's propagator (src/propagator_Ac2.cpp:366-393):
+free-surface boundary on prc, then the fused Laplacian + leapfrog on prn, over
+the 1000^3 inner grid (eigen mode 2,2,2 initial state, dt = CFL limit).  Grids
+are resident in HBM before timing.  Prints one JSON line.
+
+  value      GPoint/s = inner points x steps / wall time of the timed steps
+  roofline   the fused step kernel: 16 algorithmic bytes per inner point
+             (read prc, prn, coef; write prn) / its average HIP-event duration
+  cpu_baseline  the oracle's CacheBlk restatement on this host's cores, on a
+             time-bounded sample of the same workload
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Ac2 propagator GPoint/s & %HBM roofline; adjoint gradient s/shot vs plain"
+HBM_PEAK_GBS = 8000.0
+PI_REF = 3.1415926535897
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_ac2_fp32_o8_1000.json")
+
+
+def log(msg):
+    sys.stderr.write(f"[bench] {msg}\n")
+    sys.stderr.flush()
+
+
+def propa_scalars(n, order):
+    """stableDt / dt of Propagator_Ac2::initialize in float32 (ratioCFL 1)."""
+    import numpy as np
+
+    f32 = np.float32
+    # sum |A_j| over the Myfloat2 (float) coefficients, getCFL (src/propagator_Ac2.cpp:410-439)
+    table = {8: [-205 / 72, 8 / 5, -1 / 5, 8 / 315, -1 / 560], 4: [-5 / 2, 4 / 3, -1 / 12],
+             16: [-1077749 / 352800, 16 / 9, -14 / 45, 112 / 1485, -7 / 396, 112 / 32175, -2 / 3861,
+                  16 / 315315, -1 / 411840],
+             12: [-5369 / 1800, 12 / 7, -15 / 56, 10 / 189, -1 / 112, 2 / 1925, -1 / 16632]}[order]
+    s2 = abs(float(f32(table[0]))) + 2 * sum(abs(float(f32(a))) for a in table[1:])
+    cfl = f32(math.sqrt(4.0 / (3 * s2)))
+    d = f32(1.0 / (n - 1))
+    stable = f32(f32(cfl * d) / f32(1.0))
+    return float(stable)
+
+
+def eigen_tables(p, modes):
+    from hpcscan_amd.layout import unit_origin
+
+    r = p.all_points()
+    tabs = []
+    for a in range(3):
+        o = unit_origin(p, a)
+        tabs.append([math.sin((o + i * p.d[a]) * (PI_REF * modes[a])) for i in range(r[2 * a], r[2 * a + 1] + 1)])
+    return tabs
+
+
+def gpu_leg(args):
+    import numpy as np
+
+    from hpcscan_amd import capi
+    from hpcscan_amd.layout import make_layout
+
+    capi.init(0)
+    capi.set_kernel({"auto": capi.HSX_KERNEL_AUTO, "ref": capi.HSX_KERNEL_REF,
+                     "tiled": capi.HSX_KERNEL_TILED}[args.kernel])
+    info = capi.device_info(0)
+    n, order = args.n, args.order
+    p = make_layout(3, (n, n, n), order)
+    g = capi.hsx_grid(p)
+    npt = p.n_inner_points()
+    log(f"device {info['name']} ({info['arch']}, {info['compute_units']} CUs); grid {p.n1}x{p.n2}x{p.n3}")
+
+    dt = np.float32(propa_scalars(n, order))
+    modes = (2, 2, 2)
+    gamma = math.sqrt(12)
+    tabs = eigen_tables(p, modes)
+    r_all, r_in = p.all_points(), p.inner()
+    nbytes = p.npoint * 4
+    prn, prc, coef = (capi.DeviceArray(nbytes) for _ in range(3))
+    full = [0, p.n1 - 1, 0, p.n2 - 1, 0, p.n3 - 1]
+    for x in (prn, prc, coef):
+        capi.fill_const(g, x, full, 0.0)
+    capi.fill_const(g, coef, r_in, float(np.float32(dt * dt)))
+    capi.fill_separable(g, prn, r_all, *tabs, math.sin(PI_REF * float(np.float32(-2.0 * dt)) * gamma))
+    capi.fill_separable(g, prc, r_all, *tabs, math.sin(PI_REF * float(np.float32(-dt)) * gamma))
+    capi.sync()
+
+    def step(a, b, ev=None):
+        capi.free_surface(g, b)
+        if ev:
+            ev[0].record()
+        capi.ac2_step(g, a, b, coef)
+        if ev:
+            ev[1].record()
+
+    a, b = prn, prc
+    for _ in range(args.warmup):
+        step(a, b)
+        a, b = b, a
+    capi.sync()
+    log(f"warmup {args.warmup} steps done")
+
+    events = [(capi.Event(), capi.Event()) for _ in range(args.steps)]
+    capi.sync()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(a, b, events[k])
+        a, b = b, a
+    capi.sync()
+    t1 = time.perf_counter()
+    kern_ms = [e0.elapsed_ms(e1) for e0, e1 in events]
+    wall = t1 - t0
+    log(f"timed {args.steps} steps in {wall:.3f} s")
+
+    # sanity: the wavefield is finite and of the expected size
+    s = capi.sum_abs(g, b, r_in)
+    if not math.isfinite(s) or s <= 0:
+        raise RuntimeError(f"wavefield is not finite after the run (sum|p| = {s})")
+
+    avg_kernel_s = sum(kern_ms) / len(kern_ms) / 1e3
+    alg_bytes = 16 * npt
+    achieved = alg_bytes / avg_kernel_s / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE) and (n, order, args.kernel) == (1000, 8, "auto"):
+        try:
+            traffic = json.load(open(TRAFFIC_FILE)).get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+    return {
+        "value": npt * args.steps / wall / 1e9,
+        "ms_per_step": wall / args.steps * 1e3,
+        "kernel_ms": sum(kern_ms) / len(kern_ms),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "npt": npt, "device": info, "dt": float(dt),
+    }
+
+
+def cpu_leg(args):
+    """Oracle CacheBlk (cb 9999/4/16, the reference defaults) on the host cores,
+    same grid and step, stopped after about args.cpu_seconds of stepping."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+
+    n, order = args.n, args.order
+    g = po.layout(3, [n, n, n], order)
+    d = [1.0 / (n - 1)] * 3
+    iv = po.inv2(d, np.float32)
+    prn = np.zeros(g.npoint, np.float32)
+    prc = np.zeros(g.npoint, np.float32)
+    coef = np.zeros(g.npoint, np.float32)
+    po.fill_const(g, coef, po.INNER_POINTS, 1e-7)
+    rng = np.random.default_rng(0)
+    prc[:] = rng.standard_normal(g.npoint, dtype=np.float32)
+    s = po.sched(1, (9999, 4, 16))
+    steps, t_used = 0, 0.0
+    while (t_used < args.cpu_seconds or steps < 2) and steps < 1000:
+        t0 = time.perf_counter()
+        po.free_surface(g, prc)
+        po.pressure_fd(g, prn, prc, coef, iv, s)
+        t_used += time.perf_counter() - t0
+        prn, prc = prc, prn
+        steps += 1
+    npt = n ** 3
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    log(f"cpu baseline: {steps} steps in {t_used:.1f} s on {cores} threads")
+    return {"value": npt * steps / t_used / 1e9, "unit": "GPoint/s", "cores": cores, "kind": "port",
+            "sample": f"{steps} steps of free surface + fused Ac2 step, {n}^3 FP32 O{order}, "
+                      f"oracle CacheBlk cb 9999/4/16, {cores} OpenMP threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=1000, help="inner points per axis")
+    ap.add_argument("--order", type=int, default=8, choices=[4, 8, 12, 16])
+    ap.add_argument("--kernel", default="auto", choices=["auto", "ref", "tiled"])
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one MI355X; this build runs one process on one GPU "
+              f"(--gpus {args.gpus}, WORLD_SIZE {world} requested). Multi-GPU sharding of the "
+              f"stencil (slabs along n3 with halo exchange) is described in DESIGN.md, not run here.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    gpu = gpu_leg(args)
+    cpu = None if args.no_cpu_baseline else cpu_leg(args)
+    npt = gpu["npt"]
+    pts_stencil = 6 * (args.order // 2) + 1
+    out = {
+        "metric": METRIC,
+        "value": round(gpu["value"], 3),
+        "unit": "GPoint/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(gpu["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"Propa Ac2Standard FP32 FD order {args.order}, {args.n}^3 inner grid, "
+                               f"eigen mode (2,2,2), free surface on all faces, dt at the CFL limit",
+                   "grid": [args.n] * 3, "fd_order": args.order, "propagator": "Ac2Standard",
+                   "inner_points": npt, "kernel": args.kernel},
+        "roofline": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in gpu["roofline"].items()},
+        "cpu_baseline": cpu,
+        "extra": {"fused_kernel_ms": round(gpu["kernel_ms"], 4),
+                  "hpcscan_apparent_GBps": round(gpu["value"] * (pts_stencil + 4) * 4, 1),
+                  "device": gpu["device"]["name"], "arch": gpu["device"]["arch"]},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

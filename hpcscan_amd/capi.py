@@ -1,0 +1,301 @@
+"""ctypes binding of libhpcscan_mi355x.so (the C ABI of include/hpcscan_mi355x.h).
+
+Used by the tests, bench.py and the autograd harness.  The library is built
+in-tree (hpcscan_amd/lib) by ``__graft_entry__.build()``; loading fails loudly
+when it is missing -- there is no host fallback for any entry point.
+
+PyTorch, when importable, is imported before the library is loaded so that
+both resolve the same HIP runtime (torch ships its own libamdhip64.so.7).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+from .layout import GridLayout
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libhpcscan_mi355x.so")
+
+HSX_OK = 0
+HSX_FP32, HSX_FP64 = 0, 1
+HSX_KERNEL_AUTO, HSX_KERNEL_REF, HSX_KERNEL_TILED = 0, 1, 2
+HSX_ARRAY_FILL, HSX_ARRAY_COPY, HSX_ARRAY_ADD, HSX_ARRAY_MULTIPLY, HSX_ARRAY_ADDTO = range(5)
+ALL_FACES = 0x3F
+
+try:  # one HIP runtime per process: let torch load it first
+    import torch  # noqa: F401
+except Exception:  # pragma: no cover - torch is optional for the C ABI
+    torch = None
+
+
+class HsxError(RuntimeError):
+    def __init__(self, rc: int, msg: str):
+        super().__init__(f"hsx error {rc}: {msg}")
+        self.rc = rc
+
+
+class HsxGrid(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int32), ("fd_order", ctypes.c_int32), ("storage", ctypes.c_int32),
+                ("compute", ctypes.c_int32), ("n1", ctypes.c_int64), ("n2", ctypes.c_int64),
+                ("n3", ctypes.c_int64), ("inner", ctypes.c_int64 * 6), ("halo1", ctypes.c_int64 * 6),
+                ("halo2", ctypes.c_int64 * 6), ("d", ctypes.c_double * 3)]
+
+
+class HsxDeviceProps(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 128), ("arch", ctypes.c_char * 64),
+                ("compute_units", ctypes.c_int32), ("clock_mhz", ctypes.c_int32),
+                ("mem_clock_mhz", ctypes.c_int32), ("bus_width_bits", ctypes.c_int32),
+                ("hbm_bytes", ctypes.c_int64), ("lds_bytes_per_block", ctypes.c_int32),
+                ("l2_bytes", ctypes.c_int32)]
+
+
+# every entry point declared in include/hpcscan_mi355x.h: name -> (restype, argtypes)
+_vp, _i, _i64, _d, _sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_size_t
+_P = ctypes.POINTER
+_G = _P(HsxGrid)
+_R = _P(ctypes.c_int64)
+SIGNATURES = {
+    "hsx_abi_version": (_i, []),
+    "hsx_last_error": (ctypes.c_char_p, []),
+    "hsx_init": (_i, [_i]),
+    "hsx_device_count": (_i, [_P(_i)]),
+    "hsx_device_info": (_i, [_i, _P(HsxDeviceProps)]),
+    "hsx_malloc": (_i, [_P(_vp), _sz]),
+    "hsx_free": (_i, [_vp]),
+    "hsx_memcpy_h2d": (_i, [_vp, _vp, _sz, _vp]),
+    "hsx_memcpy_d2h": (_i, [_vp, _vp, _sz, _vp]),
+    "hsx_memcpy_d2d": (_i, [_vp, _vp, _sz, _vp]),
+    "hsx_stream_create": (_i, [_P(_vp)]),
+    "hsx_stream_destroy": (_i, [_vp]),
+    "hsx_stream_sync": (_i, [_vp]),
+    "hsx_event_create": (_i, [_P(_vp)]),
+    "hsx_event_destroy": (_i, [_vp]),
+    "hsx_event_record": (_i, [_vp, _vp]),
+    "hsx_event_elapsed_ms": (_i, [_vp, _vp, _P(ctypes.c_float)]),
+    "hsx_set_kernel": (_i, [_i]),
+    "hsx_get_kernel": (_i, []),
+    "hsx_fill_const": (_i, [_G, _vp, _R, _d, _vp]),
+    "hsx_fill_separable": (_i, [_G, _vp, _R, _P(_d), _P(_d), _P(_d), _d, _vp]),
+    "hsx_array_op": (_i, [_G, _i, _vp, _vp, _vp, _d, _vp]),
+    "hsx_free_surface": (_i, [_G, _vp, _i, _vp]),
+    "hsx_fd_d2": (_i, [_G, _i, _vp, _vp, _vp]),
+    "hsx_laplacian": (_i, [_G, _vp, _vp, _vp]),
+    "hsx_update_pressure": (_i, [_G, _vp, _vp, _vp, _vp, _vp]),
+    "hsx_ac2_step": (_i, [_G, _vp, _vp, _vp, _vp]),
+    "hsx_sum_abs": (_i, [_G, _vp, _R, _P(_d), _vp]),
+    "hsx_sum_abs_diff": (_i, [_G, _vp, _vp, _R, _P(_d), _vp]),
+    "hsx_l1_sums": (_i, [_G, _vp, _vp, _R, _P(_d), _vp]),
+    "hsx_max_err": (_i, [_G, _vp, _vp, _R, _d, _P(_d), _vp]),
+    "hsx_min_max": (_i, [_G, _vp, _R, _P(_d), _vp]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load the library once; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    return (lib().hsx_last_error() or b"").decode(errors="replace")
+
+
+def check(rc: int):
+    if rc != HSX_OK:
+        raise HsxError(rc, last_error())
+
+
+def init(device: int = 0):
+    check(lib().hsx_init(device))
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    check(lib().hsx_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def device_info(device: int = 0) -> dict:
+    p = HsxDeviceProps()
+    check(lib().hsx_device_info(device, ctypes.byref(p)))
+    return {"name": p.name.decode(), "arch": p.arch.decode(), "compute_units": p.compute_units,
+            "clock_mhz": p.clock_mhz, "hbm_bytes": p.hbm_bytes, "l2_bytes": p.l2_bytes}
+
+
+def set_kernel(family: int):
+    check(lib().hsx_set_kernel(family))
+
+
+def hsx_grid(g: GridLayout, compute_fp64: Optional[bool] = None) -> HsxGrid:
+    """C descriptor of a layout (storage from g.fp64, compute defaults to storage)."""
+    c = HsxGrid()
+    c.dim, c.fd_order = g.dim, g.fd_order
+    c.storage = HSX_FP64 if g.fp64 else HSX_FP32
+    cf = g.fp64 if compute_fp64 is None else compute_fp64
+    c.compute = HSX_FP64 if cf else HSX_FP32
+    c.n1, c.n2, c.n3 = g.n1, g.n2, g.n3
+    for a, x in enumerate(g.axes):
+        c.inner[2 * a], c.inner[2 * a + 1] = x.inner_start, x.inner_end
+        c.halo1[2 * a], c.halo1[2 * a + 1] = x.halo1_start, x.halo1_end
+        c.halo2[2 * a], c.halo2[2 * a + 1] = x.halo2_start, x.halo2_end
+        c.d[a] = g.d[a]
+    return c
+
+
+def rng(r: Sequence[int]):
+    return (ctypes.c_int64 * 6)(*[int(v) for v in r])
+
+
+class DeviceArray:
+    """A device buffer owned by the library allocator."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        check(lib().hsx_malloc(ctypes.byref(p), self.nbytes))
+        self.ptr = p
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.nbytes)
+        d.upload(a)
+        return d
+
+    def upload(self, a: np.ndarray, stream=None):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes == self.nbytes
+        check(lib().hsx_memcpy_h2d(self.ptr, a.ctypes.data_as(ctypes.c_void_p), a.nbytes, stream))
+
+    def download(self, dtype, count: Optional[int] = None, stream=None) -> np.ndarray:
+        count = self.nbytes // np.dtype(dtype).itemsize if count is None else count
+        out = np.empty(count, dtype=dtype)
+        check(lib().hsx_memcpy_d2h(out.ctypes.data_as(ctypes.c_void_p), self.ptr, out.nbytes, stream))
+        return out
+
+    def free(self):
+        if self.ptr is not None and self.ptr.value:
+            lib().hsx_free(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def dptr(x) -> ctypes.c_void_p:
+    if isinstance(x, DeviceArray):
+        return x.ptr
+    if torch is not None and isinstance(x, torch.Tensor):
+        return ctypes.c_void_p(x.data_ptr())
+    if x is None:
+        return None
+    return ctypes.c_void_p(int(x))
+
+
+# ---- thin wrappers --------------------------------------------------------------
+def fill_const(g: HsxGrid, u, r, val, stream=None):
+    check(lib().hsx_fill_const(ctypes.byref(g), dptr(u), rng(r), float(val), stream))
+
+
+def fill_separable(g: HsxGrid, u, r, t1, t2, t3, amp, stream=None):
+    t = [np.ascontiguousarray(x, dtype=np.float64) for x in (t1, t2, t3)]
+    ptrs = [x.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for x in t]
+    check(lib().hsx_fill_separable(ctypes.byref(g), dptr(u), rng(r), *ptrs, float(amp), stream))
+
+
+def array_op(g: HsxGrid, op, w, u=None, v=None, val=0.0, stream=None):
+    check(lib().hsx_array_op(ctypes.byref(g), op, dptr(w), dptr(u), dptr(v), float(val), stream))
+
+
+def free_surface(g: HsxGrid, u, face_mask=ALL_FACES, stream=None):
+    check(lib().hsx_free_surface(ctypes.byref(g), dptr(u), face_mask, stream))
+
+
+def fd_d2(g: HsxGrid, axis, u, w, stream=None):
+    check(lib().hsx_fd_d2(ctypes.byref(g), axis, dptr(u), dptr(w), stream))
+
+
+def laplacian(g: HsxGrid, u, w, stream=None):
+    check(lib().hsx_laplacian(ctypes.byref(g), dptr(u), dptr(w), stream))
+
+
+def update_pressure(g: HsxGrid, prn, prc, coef, lapla, stream=None):
+    check(lib().hsx_update_pressure(ctypes.byref(g), dptr(prn), dptr(prc), dptr(coef), dptr(lapla), stream))
+
+
+def ac2_step(g: HsxGrid, prn, prc, coef, stream=None):
+    check(lib().hsx_ac2_step(ctypes.byref(g), dptr(prn), dptr(prc), dptr(coef), stream))
+
+
+def sum_abs(g: HsxGrid, u, r, stream=None) -> float:
+    out = ctypes.c_double()
+    check(lib().hsx_sum_abs(ctypes.byref(g), dptr(u), rng(r), ctypes.byref(out), stream))
+    return out.value
+
+
+def sum_abs_diff(g: HsxGrid, u, v, r, stream=None) -> float:
+    out = ctypes.c_double()
+    check(lib().hsx_sum_abs_diff(ctypes.byref(g), dptr(u), dptr(v), rng(r), ctypes.byref(out), stream))
+    return out.value
+
+
+def l1_sums(g: HsxGrid, u, v, r, stream=None):
+    out = (ctypes.c_double * 2)()
+    check(lib().hsx_l1_sums(ctypes.byref(g), dptr(u), dptr(v), rng(r), out, stream))
+    return out[0], out[1]
+
+
+def max_err(g: HsxGrid, u, v, r, max_err_float, stream=None) -> float:
+    out = ctypes.c_double()
+    check(lib().hsx_max_err(ctypes.byref(g), dptr(u), dptr(v), rng(r), float(max_err_float),
+                            ctypes.byref(out), stream))
+    return out.value
+
+
+def min_max(g: HsxGrid, u, r, stream=None):
+    out = (ctypes.c_double * 2)()
+    check(lib().hsx_min_max(ctypes.byref(g), dptr(u), rng(r), out, stream))
+    return out[0], out[1]
+
+
+def sync(stream=None):
+    check(lib().hsx_stream_sync(stream))
+
+
+class Event:
+    def __init__(self):
+        p = ctypes.c_void_p()
+        check(lib().hsx_event_create(ctypes.byref(p)))
+        self.ptr = p
+
+    def record(self, stream=None):
+        check(lib().hsx_event_record(self.ptr, stream))
+
+    def elapsed_ms(self, later: "Event") -> float:
+        ms = ctypes.c_float()
+        check(lib().hsx_event_elapsed_ms(self.ptr, later.ptr, ctypes.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        try:
+            lib().hsx_event_destroy(self.ptr)
+        except Exception:
+            pass

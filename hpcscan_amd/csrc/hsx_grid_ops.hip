@@ -1,0 +1,438 @@
+// Grid operations of the C ABI other than the stencils: fills, array
+// operations, the free-surface boundary, the leapfrog update and the
+// double-precision reductions.
+//
+// These are HBM-streaming passes: one thread per value, rows of the index box
+// mapped to waves so that n1 (contiguous) runs across the 64 lanes.
+#include <cfloat>
+#include <mutex>
+
+#include "hsx_internal.h"
+
+namespace hsx {
+
+constexpr int ROW_THREADS = 64;   // one wave per row
+constexpr int ROWS_PER_BLOCK = 4; // 256 threads
+
+static dim3 row_grid(const Box &b)
+{
+    const int64_t rows = b.n2() * b.n3();
+    int64_t nb = (rows + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK;
+    return dim3((unsigned)(nb > 65535 * 16 ? 65535 * 16 : nb));
+}
+
+// Visit every point of box b: f(flat index, i1, i2, i3)
+template <typename F>
+__device__ inline void for_box(const Box &b, int64_t n1, int64_t n12, F &&f)
+{
+    const int64_t N1 = b.n1(), N2 = b.n2(), rows = N2 * b.n3();
+    for (int64_t row = (int64_t)blockIdx.x * ROWS_PER_BLOCK + threadIdx.y; row < rows;
+         row += (int64_t)gridDim.x * ROWS_PER_BLOCK) {
+        const int64_t i2 = b.s2 + row % N2, i3 = b.s3 + row / N2;
+        const int64_t base = i2 * n1 + i3 * n12;
+        for (int64_t k = threadIdx.x; k < N1; k += ROW_THREADS) {
+            const int64_t i1 = b.s1 + k;
+            f(base + i1, i1, i2, i3);
+        }
+    }
+}
+
+// ---- fills (src/grid.cpp:1650-1826) -----------------------------------------
+template <typename TS>
+__global__ void __launch_bounds__(256) k_fill_const(TS *u, Box b, int64_t n1, int64_t n12, TS val)
+{
+    for_box(b, n1, n12, [&](int64_t ii, int64_t, int64_t, int64_t) { u[ii] = val; });
+}
+
+// amp * v1 * v2 * v3 left to right in double, then rounded to the storage type
+template <typename TS>
+__global__ void __launch_bounds__(256) k_fill_sep(TS *u, Box b, int64_t n1, int64_t n12, const double *t1,
+                                                  const double *t2, const double *t3, double amp)
+{
+    for_box(b, n1, n12, [&](int64_t ii, int64_t i1, int64_t i2, int64_t i3) {
+        u[ii] = (TS)(amp * t1[i1 - b.s1] * t2[i2 - b.s2] * t3[i3 - b.s3]);
+    });
+}
+
+// ---- array operations (src/grid.cpp:1830-1920) --------------------------------
+template <typename TS>
+__global__ void __launch_bounds__(256) k_array(int op, TS *w, const TS *u, const TS *v, TS val, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        switch (op) {
+        case HSX_ARRAY_FILL: w[i] = val; break;
+        case HSX_ARRAY_COPY: w[i] = u[i]; break;
+        case HSX_ARRAY_ADD: w[i] = u[i] + v[i]; break;
+        case HSX_ARRAY_MULTIPLY: w[i] = u[i] * v[i]; break;
+        default: w[i] = w[i] + u[i]; break;
+        }
+    }
+}
+
+// ---- free surface, one face (src/grid.cpp:3065-3230) ---------------------------
+// For every row of the two other axes: u[inner] = 0 and u[i] = -u[2*inner - i]
+// for i in the halo range.  Sources are inner points at distance >= 1 from the
+// zeroed plane and no halo point is a source, so one face is a parallel map;
+// faces run one after the other, in the reference order.
+struct Face {
+    int axis;
+    int64_t hs, he, inner;      // halo range and zeroed plane along 'axis'
+    int64_t os, oe, bs, be;     // ranges of the outer / inner other axes
+    int64_t so, sb, sa;         // strides of outer, inner-other and face axes
+};
+
+template <typename TS>
+__global__ void __launch_bounds__(256) k_free_surface(TS *u, Face f)
+{
+    const int64_t H = f.he - f.hs + 2;           // zero + halo points per row
+    const int64_t NB = f.be - f.bs + 1, NO = f.oe - f.os + 1;
+    const int64_t total = H * NB * NO;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        int64_t k, ib, io;
+        if (f.axis == 0) {                        // face axis contiguous: k fastest
+            k = t % H;
+            ib = (t / H) % NB;
+            io = t / (H * NB);
+        } else {                                  // i1 contiguous: i1 fastest
+            ib = t % NB;
+            k = (t / NB) % H;
+            io = t / (NB * H);
+        }
+        const int64_t base = (f.os + io) * f.so + (f.bs + ib) * f.sb;
+        if (k == 0) {
+            u[base + f.inner * f.sa] = (TS)0.0;
+        } else {
+            const int64_t i = f.hs + k - 1;
+            u[base + i * f.sa] = -u[base + (2 * f.inner - i) * f.sa];
+        }
+    }
+}
+
+// ---- leapfrog update of the split propagator (src/grid.cpp:4579-4610) -----------
+// prn = TWO*prc - prn + coef*lapla; coef*lapla is a product of two storage values
+template <typename TS, typename TC>
+__global__ void __launch_bounds__(256) k_update_pressure(TS *prn, const TS *prc, const TS *coef, const TS *lapla,
+                                                         Box b, int64_t n1, int64_t n12)
+{
+    const TC two = (TC)2.0;
+    for_box(b, n1, n12, [&](int64_t ii, int64_t, int64_t, int64_t) {
+        const TS cl = coef[ii] * lapla[ii];
+        prn[ii] = (TS)(two * (TC)prc[ii] - (TC)prn[ii] + (TC)cl);
+    });
+}
+
+// ---- reductions (src/grid.cpp:2006-2370) ----------------------------------------
+enum RedOp { RED_SUM_ABS, RED_SUM_ABS_DIFF, RED_L1, RED_MAX_ERR, RED_MIN_MAX };
+
+template <int OP>
+struct RedInit {
+    static __device__ double a() { return OP == RED_MAX_ERR ? -FLT_MAX : OP == RED_MIN_MAX ? FLT_MAX : 0.0; }
+    static __device__ double b() { return OP == RED_MIN_MAX ? -FLT_MAX : 0.0; }
+};
+
+template <int OP>
+__device__ inline void red_combine(double &a, double &b, double a2, double b2)
+{
+    if (OP == RED_MAX_ERR) {
+        a = a2 > a ? a2 : a;
+    } else if (OP == RED_MIN_MAX) {
+        a = a2 < a ? a2 : a;
+        b = b2 > b ? b2 : b;
+    } else {
+        a += a2;
+        b += b2;
+    }
+}
+
+template <int OP>
+__device__ inline void block_reduce(double &a, double &b)
+{
+    __shared__ double sa[8], sb[8];
+    for (int off = 32; off > 0; off >>= 1) {
+        double a2 = __shfl_down(a, off, 64), b2 = __shfl_down(b, off, 64);
+        red_combine<OP>(a, b, a2, b2);
+    }
+    const int lane = (threadIdx.y * blockDim.x + threadIdx.x) & 63;
+    const int wave = (threadIdx.y * blockDim.x + threadIdx.x) >> 6;
+    const int nwave = (blockDim.x * blockDim.y + 63) >> 6;
+    if (lane == 0) {
+        sa[wave] = a;
+        sb[wave] = b;
+    }
+    __syncthreads();
+    if (wave == 0) {
+        a = lane < nwave ? sa[lane] : RedInit<OP>::a();
+        b = lane < nwave ? sb[lane] : RedInit<OP>::b();
+        for (int off = 4; off > 0; off >>= 1) {
+            double a2 = __shfl_down(a, off, 64), b2 = __shfl_down(b, off, 64);
+            red_combine<OP>(a, b, a2, b2);
+        }
+    }
+}
+
+template <typename TS, int OP>
+__global__ void __launch_bounds__(256) k_reduce(const TS *u, const TS *v, Box b, int64_t n1, int64_t n12,
+                                                float maxerr, double *part)
+{
+    double a = RedInit<OP>::a(), c = RedInit<OP>::b();
+    // min/max run in the storage type, as the reference's Myfloat accumulator does
+    TS mn = (TS)FLT_MAX, mx = (TS)-FLT_MAX;
+    for_box(b, n1, n12, [&](int64_t ii, int64_t, int64_t, int64_t) {
+        if (OP == RED_SUM_ABS) {
+            a += fabs((double)u[ii]);
+        } else if (OP == RED_SUM_ABS_DIFF) {
+            a += fabs((double)u[ii] - (double)v[ii]);
+        } else if (OP == RED_L1) {
+            a += fabs((double)u[ii] - (double)v[ii]);
+            c += fabs((double)v[ii]);
+        } else if (OP == RED_MAX_ERR) {
+            const double d = fabs((double)u[ii] - (double)v[ii]);
+            const double e = fabsf((float)v[ii]) < maxerr ? d : d / fabs((double)v[ii]);
+            if (e > a) a = e;
+        } else {
+            const TS x = u[ii];
+            if (x < mn) mn = x;
+            if (x > mx) mx = x;
+        }
+    });
+    if (OP == RED_MIN_MAX) {
+        a = (double)mn;
+        c = (double)mx;
+    }
+    block_reduce<OP>(a, c);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        part[blockIdx.x] = a;
+        part[gridDim.x + blockIdx.x] = c;
+    }
+}
+
+template <int OP>
+__global__ void __launch_bounds__(256) k_reduce_final(const double *part, int nb, double *out)
+{
+    double a = RedInit<OP>::a(), c = RedInit<OP>::b();
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) red_combine<OP>(a, c, part[i], part[nb + i]);
+    block_reduce<OP>(a, c);
+    if (threadIdx.x == 0) {
+        out[0] = a;
+        out[1] = c;
+    }
+}
+
+static std::mutex g_red_mu;   // the partial-sum scratch is shared by all callers
+
+template <int OP>
+static int run_reduce(const hsx_grid *g, const void *u, const void *v, const int64_t r[6], float maxerr,
+                      double out[2], void *stream)
+{
+    int rc = check_grid(g);
+    if (!rc) rc = check_range(g, r);
+    if (!rc) rc = check_ptr(u, "u");
+    if (!rc && OP != RED_SUM_ABS && OP != RED_MIN_MAX) rc = check_ptr(v, "v");
+    if (!rc && !out) rc = fail(HSX_ERR_INVALID, "null out");
+    if (!rc) rc = ensure_init();
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(g_red_mu);
+    const Box b = box_of(r);
+    const int64_t rows = b.n2() * b.n3();
+    int nb = (int)((rows + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK);
+    if (nb > 1024) nb = 1024;
+    double *part = nullptr, *res = nullptr;
+    if ((rc = scratch_get(SCRATCH_PARTIALS, sizeof(double) * 2 * nb, (void **)&part))) return rc;
+    if ((rc = scratch_get(SCRATCH_RESULT, sizeof(double) * 2, (void **)&res))) return rc;
+    hipStream_t s = stream_of(stream);
+    rc = dispatch_types(g, [&](auto ts, auto) -> int {
+        using TS = decltype(ts);
+        hipLaunchKernelGGL((k_reduce<TS, OP>), dim3(nb), dim3(ROW_THREADS, ROWS_PER_BLOCK), 0, s, (const TS *)u,
+                           (const TS *)v, b, g->n1, g->n1 * g->n2, maxerr, part);
+        HSX_LAUNCHED("k_reduce");
+        return HSX_OK;
+    });
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_reduce_final<OP>), dim3(1), dim3(256), 0, s, part, nb, res);
+    HSX_LAUNCHED("k_reduce_final");
+    HSX_HIP(hipMemcpyAsync(out, res, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
+    HSX_HIP(hipStreamSynchronize(s));
+    return HSX_OK;
+}
+
+static int common_checks(const hsx_grid *g, std::initializer_list<std::pair<const void *, const char *>> ptrs)
+{
+    int rc = check_grid(g);
+    for (auto &p : ptrs)
+        if (!rc) rc = check_ptr(p.first, p.second);
+    if (!rc) rc = ensure_init();
+    return rc;
+}
+
+} // namespace hsx
+
+using namespace hsx;
+
+extern "C" {
+
+int hsx_fill_const(const hsx_grid *g, void *u, const int64_t range[6], double val, void *stream)
+{
+    int rc = common_checks(g, {{u, "u"}});
+    if (!rc) rc = check_range(g, range);
+    if (rc) return rc;
+    const Box b = box_of(range);
+    hipStream_t s = stream_of(stream);
+    return dispatch_types(g, [&](auto ts, auto) -> int {
+        using TS = decltype(ts);
+        hipLaunchKernelGGL(k_fill_const<TS>, row_grid(b), dim3(ROW_THREADS, ROWS_PER_BLOCK), 0, s, (TS *)u, b,
+                           g->n1, g->n1 * g->n2, (TS)val);
+        HSX_LAUNCHED("k_fill_const");
+        return HSX_OK;
+    });
+}
+
+int hsx_fill_separable(const hsx_grid *g, void *u, const int64_t range[6], const double *t1,
+                       const double *t2, const double *t3, double amp, void *stream)
+{
+    int rc = common_checks(g, {{u, "u"}, {t1, "t1"}, {t2, "t2"}, {t3, "t3"}});
+    if (!rc) rc = check_range(g, range);
+    if (rc) return rc;
+    const Box b = box_of(range);
+    const size_t l1 = b.n1(), l2 = b.n2(), l3 = b.n3();
+    double *tab = nullptr;
+    static std::mutex mu;   // the table staging buffer is shared
+    std::lock_guard<std::mutex> lk(mu);
+    if ((rc = scratch_get(SCRATCH_TABLES, sizeof(double) * (l1 + l2 + l3), (void **)&tab))) return rc;
+    hipStream_t s = stream_of(stream);
+    HSX_HIP(hipMemcpyAsync(tab, t1, sizeof(double) * l1, hipMemcpyHostToDevice, s));
+    HSX_HIP(hipMemcpyAsync(tab + l1, t2, sizeof(double) * l2, hipMemcpyHostToDevice, s));
+    HSX_HIP(hipMemcpyAsync(tab + l1 + l2, t3, sizeof(double) * l3, hipMemcpyHostToDevice, s));
+    rc = dispatch_types(g, [&](auto ts, auto) -> int {
+        using TS = decltype(ts);
+        hipLaunchKernelGGL(k_fill_sep<TS>, row_grid(b), dim3(ROW_THREADS, ROWS_PER_BLOCK), 0, s, (TS *)u, b,
+                           g->n1, g->n1 * g->n2, tab, tab + l1, tab + l1 + l2, amp);
+        HSX_LAUNCHED("k_fill_sep");
+        return HSX_OK;
+    });
+    if (rc) return rc;
+    // the staging buffer is reused by the next call: finish with it here
+    HSX_HIP(hipStreamSynchronize(s));
+    return HSX_OK;
+}
+
+int hsx_array_op(const hsx_grid *g, int op, void *w, const void *u, const void *v, double val, void *stream)
+{
+    int rc = common_checks(g, {{w, "w"}});
+    if (rc) return rc;
+    if (op < HSX_ARRAY_FILL || op > HSX_ARRAY_ADDTO) return fail(HSX_ERR_INVALID, "array op %d", op);
+    if (op != HSX_ARRAY_FILL && (rc = check_ptr(u, "u"))) return rc;
+    if ((op == HSX_ARRAY_ADD || op == HSX_ARRAY_MULTIPLY) && (rc = check_ptr(v, "v"))) return rc;
+    const int64_t n = g->n1 * g->n2 * g->n3;
+    int64_t nb = (n + 255) / 256;
+    if (nb > 8192) nb = 8192;
+    hipStream_t s = stream_of(stream);
+    return dispatch_types(g, [&](auto ts, auto) -> int {
+        using TS = decltype(ts);
+        hipLaunchKernelGGL(k_array<TS>, dim3((unsigned)nb), dim3(256), 0, s, op, (TS *)w, (const TS *)u,
+                           (const TS *)v, (TS)val, n);
+        HSX_LAUNCHED("k_array");
+        return HSX_OK;
+    });
+}
+
+int hsx_free_surface(const hsx_grid *g, void *u, int face_mask, void *stream)
+{
+    int rc = common_checks(g, {{u, "u"}});
+    if (rc) return rc;
+    hipStream_t s = stream_of(stream);
+    const int64_t st[3] = {1, g->n1, g->n1 * g->n2};
+    for (int axis = 0; axis < g->dim; axis++) {
+        for (int side = 0; side < 2; side++) {
+            if (!(face_mask & (1 << (2 * axis + side)))) continue;
+            const int64_t *hr = side == 0 ? g->halo1 : g->halo2;
+            Face f;
+            f.axis = axis;
+            f.hs = hr[2 * axis];
+            f.he = hr[2 * axis + 1];
+            f.inner = side == 0 ? f.he + 1 : f.hs - 1;
+            // other axes: outer = the higher one, inner-other = the lower one
+            const int oa = axis == 2 ? 1 : 2, ob = axis == 0 ? 1 : 0;
+            f.os = g->inner[2 * oa];
+            f.oe = g->inner[2 * oa + 1];
+            f.bs = g->inner[2 * ob];
+            f.be = g->inner[2 * ob + 1];
+            f.so = st[oa];
+            f.sb = st[ob];
+            f.sa = st[axis];
+            // mirror sources must stay inside the allocation
+            const int64_t n_ax = axis == 0 ? g->n1 : axis == 1 ? g->n2 : g->n3;
+            if (2 * f.inner - f.hs >= n_ax || 2 * f.inner - f.he < 0 || f.inner < 0 || f.inner >= n_ax)
+                return fail(HSX_ERR_INVALID, "free surface: mirror of axis %d face %d outside the grid", axis + 1,
+                            side + 1);
+            const int64_t total = (f.he - f.hs + 2) * (f.be - f.bs + 1) * (f.oe - f.os + 1);
+            int64_t nb = (total + 255) / 256;
+            if (nb > 16384) nb = 16384;
+            rc = dispatch_types(g, [&](auto ts, auto) -> int {
+                using TS = decltype(ts);
+                hipLaunchKernelGGL(k_free_surface<TS>, dim3((unsigned)nb), dim3(256), 0, s, (TS *)u, f);
+                HSX_LAUNCHED("k_free_surface");
+                return HSX_OK;
+            });
+            if (rc) return rc;
+        }
+    }
+    return HSX_OK;
+}
+
+int hsx_update_pressure(const hsx_grid *g, void *prn, const void *prc, const void *coef, const void *lapla,
+                        void *stream)
+{
+    int rc = common_checks(g, {{prn, "prn"}, {prc, "prc"}, {coef, "coef"}, {lapla, "lapla"}});
+    if (rc) return rc;
+    const Box b = box_of(g->inner);
+    hipStream_t s = stream_of(stream);
+    return dispatch_types(g, [&](auto ts, auto tc) -> int {
+        using TS = decltype(ts);
+        using TC = decltype(tc);
+        hipLaunchKernelGGL((k_update_pressure<TS, TC>), row_grid(b), dim3(ROW_THREADS, ROWS_PER_BLOCK), 0, s,
+                           (TS *)prn, (const TS *)prc, (const TS *)coef, (const TS *)lapla, b, g->n1,
+                           g->n1 * g->n2);
+        HSX_LAUNCHED("k_update_pressure");
+        return HSX_OK;
+    });
+}
+
+int hsx_sum_abs(const hsx_grid *g, const void *u, const int64_t range[6], double *out, void *stream)
+{
+    double r[2];
+    int rc = run_reduce<RED_SUM_ABS>(g, u, nullptr, range, 0.f, r, stream);
+    if (!rc) *out = r[0];
+    return rc;
+}
+
+int hsx_sum_abs_diff(const hsx_grid *g, const void *u, const void *v, const int64_t range[6], double *out,
+                     void *stream)
+{
+    double r[2];
+    int rc = run_reduce<RED_SUM_ABS_DIFF>(g, u, v, range, 0.f, r, stream);
+    if (!rc) *out = r[0];
+    return rc;
+}
+
+int hsx_l1_sums(const hsx_grid *g, const void *u, const void *v, const int64_t range[6], double out[2],
+                void *stream)
+{
+    return run_reduce<RED_L1>(g, u, v, range, 0.f, out, stream);
+}
+
+int hsx_max_err(const hsx_grid *g, const void *u, const void *v, const int64_t range[6], double max_err_float,
+                double *out, void *stream)
+{
+    double r[2];
+    int rc = run_reduce<RED_MAX_ERR>(g, u, v, range, (float)max_err_float, r, stream);
+    if (!rc) *out = r[0];
+    return rc;
+}
+
+int hsx_min_max(const hsx_grid *g, const void *u, const int64_t range[6], double out[2], void *stream)
+{
+    return run_reduce<RED_MIN_MAX>(g, u, nullptr, range, 0.f, out, stream);
+}
+
+} // extern "C"

@@ -1,0 +1,123 @@
+// Internal helpers shared by the HIP translation units of libhpcscan_mi355x.
+#ifndef HSX_INTERNAL_H
+#define HSX_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <type_traits>
+
+#include "hpcscan_mi355x.h"
+
+namespace hsx {
+
+// ---- errors ---------------------------------------------------------------
+int fail(int code, const char *fmt, ...);            // record message, return code
+int hip_fail(hipError_t e, const char *what);         // HSX_ERR_HIP with the HIP message
+
+#define HSX_HIP(call)                                                      \
+    do {                                                                   \
+        hipError_t e_ = (call);                                            \
+        if (e_ != hipSuccess) return ::hsx::hip_fail(e_, #call);           \
+    } while (0)
+
+// launch-error check after hipLaunchKernelGGL
+#define HSX_LAUNCHED(name)                                                 \
+    do {                                                                   \
+        hipError_t e_ = hipGetLastError();                                 \
+        if (e_ != hipSuccess) return ::hsx::hip_fail(e_, name);            \
+    } while (0)
+
+// ---- runtime state ----------------------------------------------------------
+int ensure_init(void);                 // lazily select device 0 and create the library stream
+hipStream_t stream_of(void *s);        // NULL -> library stream
+int kernel_family(void);               // HSX_KERNEL_*
+
+// device scratch owned by the library (grown on demand, freed at exit)
+int scratch_get(int slot, size_t bytes, void **ptr);
+enum { SCRATCH_TABLES = 0, SCRATCH_PARTIALS = 1, SCRATCH_RESULT = 2, SCRATCH_SLOTS = 4 };
+
+// ---- descriptor checks ----------------------------------------------------------
+int check_grid(const hsx_grid *g);
+int check_range(const hsx_grid *g, const int64_t r[6]);
+int check_ptr(const void *p, const char *name);
+
+// ---- index helpers ------------------------------------------------------------
+struct Box {                    // inclusive index box
+    int64_t s1, e1, s2, e2, s3, e3;
+    __host__ __device__ int64_t n1() const { return e1 - s1 + 1; }
+    __host__ __device__ int64_t n2() const { return e2 - s2 + 1; }
+    __host__ __device__ int64_t n3() const { return e3 - s3 + 1; }
+};
+inline Box box_of(const int64_t r[6]) { return Box{r[0], r[1], r[2], r[3], r[4], r[5]}; }
+
+// ---- coefficients in the compute type (src/fdm.h:276-507, src/grid.cpp:3251-3257) --
+template <typename TC>
+struct Coefs {
+    TC A[9];        // A0..AR
+    TC inv2[3];     // (TC)(1/d) squared in TC
+};
+double fd_coef_double(int fd_order, int j);
+template <typename TC>
+inline Coefs<TC> make_coefs(const hsx_grid *g)
+{
+    Coefs<TC> c;
+    const int R = g->fd_order / 2;
+    for (int j = 0; j < 9; j++) c.A[j] = j <= R ? (TC)fd_coef_double(g->fd_order, j) : (TC)0;
+    for (int a = 0; a < 3; a++) {
+        const TC inv = (TC)(1.0 / g->d[a]);
+        c.inv2[a] = inv * inv;
+    }
+    return c;
+}
+
+// ---- type / order dispatch ----------------------------------------------------
+template <typename F>
+inline int dispatch_types(const hsx_grid *g, F &&f)
+{
+    if (g->storage == HSX_FP32 && g->compute == HSX_FP32) return f(float(), float());
+    if (g->storage == HSX_FP64 && g->compute == HSX_FP64) return f(double(), double());
+    if (g->storage == HSX_FP32 && g->compute == HSX_FP64) return f(float(), double());
+    return fail(HSX_ERR_UNSUPPORTED, "storage/compute precision pair %d/%d not supported", g->storage,
+                g->compute);
+}
+
+template <int V>
+using IC = std::integral_constant<int, V>;
+
+template <typename F>
+inline int dispatch_R(int R, F &&f)
+{
+    switch (R) {
+    case 1: return f(IC<1>());
+    case 2: return f(IC<2>());
+    case 3: return f(IC<3>());
+    case 4: return f(IC<4>());
+    case 5: return f(IC<5>());
+    case 6: return f(IC<6>());
+    case 7: return f(IC<7>());
+    case 8: return f(IC<8>());
+    default: return fail(HSX_ERR_INVALID, "halo width %d not in 1..8", R);
+    }
+}
+
+template <typename F>
+inline int dispatch_dim(int dim, F &&f)
+{
+    switch (dim) {
+    case 1: return f(IC<1>());
+    case 2: return f(IC<2>());
+    case 3: return f(IC<3>());
+    default: return fail(HSX_ERR_INVALID, "dim %d not in 1..3", dim);
+    }
+}
+
+// tiled kernels (hsx_stencil_tiled.hip); return HSX_ERR_UNSUPPORTED when the
+// grid does not fit their assumptions so the caller uses the reference kernels
+int tiled_ac2_step(const hsx_grid *g, void *prn, const void *prc, const void *coef, hipStream_t s);
+int tiled_laplacian(const hsx_grid *g, const void *u, void *w, hipStream_t s);
+int tiled_fd_d2(const hsx_grid *g, int axis, const void *u, void *w, hipStream_t s);
+
+} // namespace hsx
+
+#endif

@@ -1,0 +1,65 @@
+// Grid_Mi355x: the MI355X backend of the Grid plugin interface.
+//
+// Selected with -testMode MI355X.  Grid values live in HBM; every compute
+// virtual of Grid (src/grid.h:49-245) runs as hand-written gfx950 kernels
+// behind the C ABI of include/hpcscan_mi355x.h and is complete on return, so
+// the callers' wall-clock timers (src/propagator_Ac2.cpp:371-389) measure it.
+// The host array grid_3d is only filled on demand (write()).
+#ifndef HPCSCAN_GRID_MI355X_H_
+#define HPCSCAN_GRID_MI355X_H_
+
+#include "grid.h"
+#include "hpcscan_mi355x.h"
+
+namespace hpcscan {
+
+class Grid_Mi355x : public Grid
+{
+public:
+    Grid_Mi355x(Grid_type);
+    Grid_Mi355x(Grid_type, Dim_type, Myint64, Myint64, Myint64);
+    ~Grid_Mi355x(void) override;
+
+    Rtn_code initializeGrid(void) override;
+    void info(void) override;
+    Rtn_code write(string) override;
+    Rtn_code write(Point_type, string) override;
+
+    Myfloat L1Err(Point_type, const Grid &) const override;
+    Myfloat maxErr(Point_type, const Grid &) const override;
+    Myfloat getMin(Point_type) override;
+    Myfloat getMax(Point_type) override;
+    Myfloat getSumAbs(Point_type) const override;
+    Myfloat getSumAbsDiff(Point_type, const Grid &) const override;
+
+    void fill(Point_type, Myfloat) override;
+    void fill(Point_type, Func_type, Func_type, Func_type, Myfloat64, Myfloat64, Myfloat64, Myfloat64) override;
+    void fillArray(Myfloat) override;
+    void copyArray(const Grid &) override;
+    void addArray(const Grid &, const Grid &) override;
+    void multiplyArray(const Grid &, const Grid &) override;
+    void addUpdateArray(const Grid &) override;
+
+    Rtn_code applyBoundaryCondition(BoundCond_type) override;
+    Rtn_code FD_D2_N1(Point_type, const Grid &, Myint) override;
+    Rtn_code FD_D2_N2(Point_type, const Grid &, Myint) override;
+    Rtn_code FD_D2_N3(Point_type, const Grid &, Myint) override;
+    Rtn_code FD_LAPLACIAN(Point_type, const Grid &, Myint) override;
+    Rtn_code updatePressure(Point_type, const Grid &, const Grid &, const Grid &) override;
+    Rtn_code computePressureWithFD(Grid &, Grid &, Myint) override;
+
+    // device copy of the grid values (npoint * sizeof(Myfloat) bytes)
+    void *d_grid;
+
+protected:
+    hsx_grid desc(void) const;
+    void range(Point_type, int64_t r[6]) const;
+    bool check(int rc, const char *what) const;
+    const Grid_Mi355x *peer(const Grid &g, const char *what) const;
+    Rtn_code toHost(void);
+    Rtn_code fdD2(int axis, const Grid &W, Myint fdOrder);
+};
+
+} // namespace hpcscan
+
+#endif

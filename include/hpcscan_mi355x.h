@@ -1,0 +1,150 @@
+/*
+ * hpcscan_mi355x.h -- C ABI of the MI355X (gfx950) stencil library.
+ *
+ * This is the device boundary of the hpcscan stencil path: every virtual of
+ * the reference's Grid plugin interface that touches grid values
+ * (src/grid.h:49-245) maps onto one entry point below, called by the C++
+ * Grid_Mi355x backend (hpcscan_amd/host/grid_Mi355x.cpp) and, for tests and
+ * benchmarks, through ctypes.  Plain pointers and sizes only: device buffers
+ * are void*, HIP streams are passed as void* (NULL = the library's stream).
+ *
+ * Conventions
+ *  - Return value: HSX_OK (0) or a negative HSX_ERR_* code; never throws,
+ *    never exits.  hsx_last_error() gives the message of the last failure of
+ *    the calling thread.
+ *  - Grid operations are asynchronous on the given stream unless noted;
+ *    reductions synchronise the stream and return their result on the host.
+ *  - Arithmetic follows the reference per-point operation order with no
+ *    fused multiply-add, so results equal the reference CPU path bit for bit
+ *    (reductions: up to the order of a double-precision sum).
+ */
+#ifndef HPCSCAN_MI355X_H
+#define HPCSCAN_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HSX_ABI_VERSION 1
+
+/* status codes */
+#define HSX_OK               0
+#define HSX_ERR_INVALID     -1   /* bad argument (sizes, ranges, order, precision) */
+#define HSX_ERR_HIP         -2   /* HIP runtime error */
+#define HSX_ERR_NOMEM       -3   /* device allocation failed */
+#define HSX_ERR_UNSUPPORTED -4   /* valid request this build does not implement */
+
+/* precisions (reference Myfloat / Myfloat2, src/type_def.h:21-43) */
+#define HSX_FP32 0
+#define HSX_FP64 1
+
+/* stencil kernel families (hsx_set_kernel) */
+#define HSX_KERNEL_AUTO  0   /* tiled kernels where they apply, else reference kernels */
+#define HSX_KERNEL_REF   1   /* one thread per point, reference loop semantics */
+#define HSX_KERNEL_TILED 2   /* LDS-tiled n3-streaming kernels (3D only) */
+
+/*
+ * Grid descriptor: the index layout of the reference Grid (src/grid.cpp:17-38,
+ * 252-832).  Flat index i1 + i2*n1 + i3*n1*n2; all ranges inclusive.
+ * Inactive axes (dim < 3) have n = 1 and all-zero ranges, as in the reference.
+ */
+typedef struct {
+    int32_t dim;            /* 1, 2 or 3 */
+    int32_t fd_order;       /* 2, 4, ..., 16 */
+    int32_t storage;        /* HSX_FP32 or HSX_FP64: type of grid values (Myfloat) */
+    int32_t compute;        /* HSX_FP32 or HSX_FP64: arithmetic type (Myfloat2) */
+    int64_t n1, n2, n3;     /* allocated points per axis */
+    int64_t inner[6];       /* i1Start,i1End,i2Start,i2End,i3Start,i3End of INNER_POINTS */
+    int64_t halo1[6];       /* per axis: Halo1Start, Halo1End (axes 1,2,3) */
+    int64_t halo2[6];       /* per axis: Halo2Start, Halo2End */
+    double d[3];            /* grid spacing per axis (inv2 = (Myfloat2)(1/d) squared) */
+} hsx_grid;
+
+typedef struct {
+    char name[128];
+    char arch[64];
+    int32_t compute_units;
+    int32_t clock_mhz;
+    int32_t mem_clock_mhz;
+    int32_t bus_width_bits;
+    int64_t hbm_bytes;
+    int32_t lds_bytes_per_block;
+    int32_t l2_bytes;
+} hsx_device_props;
+
+/* ---- runtime ---------------------------------------------------------- */
+int         hsx_abi_version(void);
+const char *hsx_last_error(void);
+int         hsx_init(int device);
+int         hsx_device_count(int *count);
+int         hsx_device_info(int device, hsx_device_props *out);
+int         hsx_malloc(void **ptr, size_t bytes);
+int         hsx_free(void *ptr);
+int         hsx_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream);
+int         hsx_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream);
+int         hsx_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
+int         hsx_stream_create(void **stream);
+int         hsx_stream_destroy(void *stream);
+int         hsx_stream_sync(void *stream);
+int         hsx_event_create(void **event);
+int         hsx_event_destroy(void *event);
+int         hsx_event_record(void *event, void *stream);
+int         hsx_event_elapsed_ms(void *start, void *stop, float *ms);
+int         hsx_set_kernel(int family);
+int         hsx_get_kernel(void);
+
+/* ---- grid filling: Grid::fill (src/grid.cpp:1650-1826) -------------------- */
+/* u[range] = val (converted to the storage type) */
+int hsx_fill_const(const hsx_grid *g, void *u, const int64_t range[6], double val, void *stream);
+/* u[i1,i2,i3] = (Myfloat)(amp * t1[i1-r0] * t2[i2-r2] * t3[i3-r4]) over range;
+ * t1/t2/t3 are HOST tables of the 1D factors in double */
+int hsx_fill_separable(const hsx_grid *g, void *u, const int64_t range[6], const double *t1,
+                       const double *t2, const double *t3, double amp, void *stream);
+
+/* ---- Grid array operations over all npoint values (src/grid.cpp:1830-1920) */
+#define HSX_ARRAY_FILL     0   /* w = val       */
+#define HSX_ARRAY_COPY     1   /* w = u         */
+#define HSX_ARRAY_ADD      2   /* w = u + v     */
+#define HSX_ARRAY_MULTIPLY 3   /* w = u * v     */
+#define HSX_ARRAY_ADDTO    4   /* w = w + u     */
+int hsx_array_op(const hsx_grid *g, int op, void *w, const void *u, const void *v, double val,
+                 void *stream);
+
+/* ---- the stencil path --------------------------------------------------- */
+/* Grid::applyBoundaryCondition(BOUND_COND_ANTI_MIRROR) (src/grid.cpp:3065-3230):
+ * free surface on the faces whose bit is set in face_mask (bit 2a = axis a
+ * Halo1 side, bit 2a+1 = Halo2 side; 0x3f = all faces, i.e. no neighbour) */
+int hsx_free_surface(const hsx_grid *g, void *u, int face_mask, void *stream);
+/* Grid::FD_D2_N1/N2/N3 (src/grid.cpp:3232-4065): w = D2 along axis (0,1,2) of u, inner points */
+int hsx_fd_d2(const hsx_grid *g, int axis, const void *u, void *w, void *stream);
+/* Grid::FD_LAPLACIAN (src/grid.cpp:4235-4577): w = Laplacian(u), inner points */
+int hsx_laplacian(const hsx_grid *g, const void *u, void *w, void *stream);
+/* Grid::updatePressure (src/grid.cpp:4579-4610): prn = 2 prc - prn + coef*lapla */
+int hsx_update_pressure(const hsx_grid *g, void *prn, const void *prc, const void *coef,
+                        const void *lapla, void *stream);
+/* Grid::computePressureWithFD (src/grid.cpp:4612-5112): prn = 2 prc - prn + coef*Lap(prc) */
+int hsx_ac2_step(const hsx_grid *g, void *prn, const void *prc, const void *coef, void *stream);
+
+/* ---- reductions: Grid::getSumAbs ... (src/grid.cpp:2006-2370) -------------
+ * All accumulate in double on the device; the result is returned on the host. */
+int hsx_sum_abs(const hsx_grid *g, const void *u, const int64_t range[6], double *out, void *stream);
+int hsx_sum_abs_diff(const hsx_grid *g, const void *u, const void *v, const int64_t range[6],
+                     double *out, void *stream);
+/* out[0] = sum |u - v|, out[1] = sum |v| (Grid::L1Err before its divide) */
+int hsx_l1_sums(const hsx_grid *g, const void *u, const void *v, const int64_t range[6],
+                double out[2], void *stream);
+/* max over the range of |u-v|/|v|, or |u-v| where |(float)v| < max_err_float */
+int hsx_max_err(const hsx_grid *g, const void *u, const void *v, const int64_t range[6],
+                double max_err_float, double *out, void *stream);
+/* out[0] = min(FLT_MAX, u...), out[1] = max(-FLT_MAX, u...) in the storage type */
+int hsx_min_max(const hsx_grid *g, const void *u, const int64_t range[6], double out[2],
+                void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

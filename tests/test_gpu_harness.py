@@ -1,0 +1,106 @@
+"""The hpcscan command line on -testMode MI355X (run with -m gpu).
+
+The product binaries (hpcscan_amd/bin) run the reference's test cases on the
+GPU; their printed results are compared with the reference's published
+output and with the oracle build of the same harness on the CPU
+(-testMode CacheBlk), line by line for the error values.
+"""
+import json
+import os
+
+import pytest
+
+from conftest import ROOT, report_value, run_bin
+from hpcscan_amd import binary
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+CPU_BIN = {"fp32": os.path.join(ROOT, "oracle", "_build", "hpcscan_cpu"),
+           "fp64": os.path.join(ROOT, "oracle", "_build", "hpcscan_cpu_fp64")}
+
+
+def _errors(out):
+    """every printed error / PASS / FAIL line, in order"""
+    keep = ("errTestCase2", "Test PASSED", ">> Test FAILED", "ntCheck", "CFL", "Stable time step",
+            "No. time steps", "Time step (s)")
+    return [l.strip() for l in out.splitlines() if l.strip().startswith(keep)]
+
+
+def test_default_propa_matches_published(tmp_path):
+    ref = json.load(open(os.path.join(GOLDEN, "propa_default_61.json")))
+    rc, out = run_bin(binary(), ["-testCase", "Propa", "-testMode", "MI355X", "-ntry", "2"], tmp_path)
+    print(out[-3000:])
+    assert rc == 0
+    for k in ("errTestCase2", "CFL", "Stable time step (s)", "Total n1", "Grid Inner Glob. (Pts)", "ntCheck",
+              "#Flop per point", "#Point in stencil", "Max. frequency (Hz)"):
+        assert report_value(out, k) == ref[k], k
+    assert ">> Test FAILED" not in out
+    assert "Mode MI355X\tENABLED" in out
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("args", [
+    ["-testCase", "Propa", "-dim", "1", "-fdOrder", "4"],
+    ["-testCase", "Propa", "-dim", "2", "-fdOrder", "12"],
+    ["-testCase", "Propa", "-dim", "3", "-fdOrder", "16"],
+    ["-testCase", "Propa", "-dim", "3", "-fdOrder", "6", "-propagator", "Ac2SplitComp"],
+    ["-testCase", "Propa", "-dim", "3", "-nt", "50", "-dt", "0.001", "-param1", "2", "-param3", "2",
+     "-snapInc", "20"],
+    ["-testCase", "Propa", "-n1", "100", "-n2", "100", "-n3", "100", "-fdOrder", "8", "-nt", "100",
+     "-ratioCFL", "0.8", "-param1", "1", "-param2", "1", "-param3", "1"],
+    ["-testCase", "FD_D2", "-dim", "3", "-fdOrder", "8"],
+    ["-testCase", "FD_D2", "-dim", "3", "-fdOrder", "16", "-n1", "27", "-n2", "36", "-n3", "41"],
+    ["-testCase", "FD_D2", "-dim", "2", "-fdOrder", "2", "-n1AddPad", "5", "-n2AddPad", "5"],
+    ["-testCase", "FD_D2", "-n1Offset", "1", "-n2Offset", "2", "-n1MulPad", "5"],
+])
+def test_same_report_as_cpu_reference(tmp_path, prec, args):
+    """Error values printed by the MI355X run equal those of the CPU run."""
+    print(" ".join(args), prec, flush=True)
+    rc1, gpu = run_bin(binary(prec), args + ["-testMode", "MI355X", "-ntry", "1"], tmp_path)
+    rc2, cpu = run_bin(CPU_BIN[prec], args + ["-testMode", "CacheBlk", "-ntry", "1"], tmp_path)
+    assert rc1 == 0 and rc2 == 0
+    assert _errors(gpu) == _errors(cpu)
+    assert ">> Test FAILED" not in gpu
+
+
+def test_validation_matrix_zero_failed(tmp_path):
+    """The stencil part of the reference's validation driver (script/testDriver.sh:79-141,
+    233-321) on -testMode MI355X, one rank: 0 FAILED."""
+    runs = []
+    for tc in ("FD_D2", "Propa"):
+        for dim in ("1", "2", "3"):
+            for order in ("2", "4", "6", "8", "10", "12", "14", "16"):
+                runs.append(["-testCase", tc, "-dim", dim, "-fdOrder", order])
+                if tc == "Propa":
+                    runs.append(["-testCase", tc, "-dim", dim, "-fdOrder", order, "-propagator", "Ac2SplitComp"])
+    for order in ("2", "8", "16"):
+        runs.append(["-testCase", "FD_D2", "-fdOrder", order, "-dim", "3", "-autoPad"])
+    runs.append(["-testCase", "Propa", "-dim", "1", "-nt", "50", "-dt", "0.001", "-fdOrder", "8", "-n1", "100",
+                 "-param1", "2", "-snapInc", "20"])
+    runs.append(["-testCase", "Propa", "-dim", "2", "-nt", "50", "-dt", "0.001", "-fdOrder", "8", "-n1", "100",
+                 "-n2", "100", "-param2", "2", "-param1", "2", "-snapInc", "20"])
+    passed = failed = 0
+    for a in runs:
+        rc, out = run_bin(binary(), a + ["-testMode", "MI355X", "-ntry", "1"], tmp_path)
+        assert rc == 0, a
+        passed += out.count("Test PASSED")
+        failed += out.count("Test FAILED")
+        assert "F A T A L" not in out, (a, out[-2000:])
+    print(f"validation matrix: {len(runs)} runs, {passed} PASSED, {failed} FAILED")
+    assert failed == 0 and passed >= len(runs)
+
+
+@pytest.mark.slow
+def test_published_1000_cube_error(tmp_path):
+    """Propa 1000^3 O8 -ratioCFL 0.5, modes 130, tmax 0.1, snapDt 0.01: the
+    reference's CPU run printed errTestCase 0.0340336 (nt 451)."""
+    args = ["-testCase", "Propa", "-testMode", "MI355X", "-ntry", "1", "-n1", "1000", "-n2", "1000", "-n3", "1000",
+            "-fdOrder", "8", "-ratioCFL", "0.5", "-tmax", "0.1", "-snapDt", "0.01", "-param1", "130",
+            "-param2", "130", "-param3", "130"]
+    rc, out = run_bin(binary(), args, tmp_path, timeout=900)
+    print(out[-2500:])
+    assert rc == 0
+    assert report_value(out, "No. time steps") == "451"
+    err = float(report_value(out, "errTestCase2"))
+    assert abs(err / 0.0340336 - 1) < 5e-5

@@ -1,0 +1,251 @@
+"""MI355X C ABI vs the CPU oracle on identical inputs (run with -m gpu).
+
+Bar: bit-exact for every stencil, fill, boundary and update (same operation
+order, no fma on either side); reductions to 1e-12 relative (double sums in a
+different order); min / max / maxErr exact.
+"""
+import numpy as np
+import pytest
+
+from hpcscan_amd.layout import make_layout
+
+pytestmark = pytest.mark.gpu
+
+PRECS = {  # name: (storage dtype, compute fp64)
+    "f32": (np.float32, False),
+    "f64": (np.float64, True),
+    "f32c64": (np.float32, True),
+}
+SHAPES = {1: [(37,)], 2: [(23, 19)], 3: [(21, 18, 17), (9, 10, 11)]}
+
+
+def _layouts(oracle, dim, n, order, dtype, **kw):
+    fp64 = dtype == np.float64
+    p = make_layout(dim, n, order, fp64=fp64, **kw)
+    o = oracle.layout(dim, n, order, storage_fp64=fp64, **kw)
+    assert p.npoint == o.npoint
+    return p, o
+
+
+def _rand(rng, n, dtype):
+    return rng.standard_normal(n).astype(dtype)
+
+
+def _same(a, b):
+    """bitwise equality up to the sign of zero"""
+    assert a.dtype == b.dtype and a.shape == b.shape
+    eq = (a == b) | (np.isnan(a) & np.isnan(b))
+    if not eq.all():
+        i = np.flatnonzero(~eq)[:5]
+        raise AssertionError(f"{(~eq).sum()} values differ, e.g. at {i}: {a[i]} vs {b[i]}")
+
+
+@pytest.fixture(params=["ref", "auto"])
+def family(request, capi):
+    capi.set_kernel(capi.HSX_KERNEL_REF if request.param == "ref" else capi.HSX_KERNEL_AUTO)
+    yield request.param
+    capi.set_kernel(capi.HSX_KERNEL_AUTO)
+
+
+def _cases():
+    out = []
+    for dim, shapes in SHAPES.items():
+        for n in shapes:
+            for order in (2, 4, 6, 8, 10, 12, 14, 16):
+                if min(n[:dim]) < order // 2 + 2:
+                    continue
+                for prec in PRECS:
+                    out.append((dim, n, order, prec))
+    return out
+
+
+@pytest.mark.parametrize("dim,n,order,prec", _cases())
+def test_stencils_bit_exact(oracle, capi, family, dim, n, order, prec):
+    dtype, c64 = PRECS[prec]
+    p, o = _layouts(oracle, dim, n, order, dtype)
+    print(f"case dim={dim} n={n} order={order} prec={prec} family={family}", flush=True)
+    rng = np.random.default_rng(1000 * dim + order)
+    g = capi.hsx_grid(p, compute_fp64=c64)
+    u = _rand(rng, p.npoint, dtype)
+    coef = (0.1 * _rand(rng, p.npoint, dtype)).astype(dtype)
+    prn0 = _rand(rng, p.npoint, dtype)
+    w0 = _rand(rng, p.npoint, dtype)
+    iv = oracle.inv2(p.d, dtype, compute64=c64)
+    du, dw = capi.DeviceArray.from_numpy(u), capi.DeviceArray.from_numpy(w0)
+    dc, dp = capi.DeviceArray.from_numpy(coef), capi.DeviceArray.from_numpy(prn0)
+
+    for axis in range(dim):
+        ref = w0.copy()
+        oracle.fd_d2(o, axis, u, ref, iv, compute64=c64)
+        dw.upload(w0)
+        capi.fd_d2(g, axis, du, dw)
+        _same(dw.download(dtype), ref)
+
+    ref = w0.copy()
+    oracle.laplacian(o, u, ref, iv, compute64=c64)
+    dw.upload(w0)
+    capi.laplacian(g, du, dw)
+    _same(dw.download(dtype), ref)
+
+    ref = prn0.copy()
+    oracle.pressure_fd(o, ref, u, coef, iv, compute64=c64)
+    capi.ac2_step(g, dp, du, dc)
+    _same(dp.download(dtype), ref)
+
+    # SplitComp update with w0 as the Laplacian grid
+    ref = prn0.copy()
+    oracle.update_pressure(o, ref, u, coef, w0, compute64=c64)
+    dp.upload(prn0)
+    dw.upload(w0)
+    capi.update_pressure(g, dp, du, dc, dw)
+    _same(dp.download(dtype), ref)
+
+
+@pytest.mark.parametrize("dim,n", [(1, (30,)), (2, (17, 20)), (3, (12, 11, 13))])
+@pytest.mark.parametrize("order", [2, 8, 16])
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_free_surface_bit_exact(oracle, capi, dim, n, order, prec):
+    dtype, _ = PRECS[prec]
+    p, o = _layouts(oracle, dim, n, order, dtype)
+    rng = np.random.default_rng(order)
+    u = _rand(rng, p.npoint, dtype)
+    ref = u.copy()
+    oracle.free_surface(o, ref)
+    d = capi.DeviceArray.from_numpy(u)
+    capi.free_surface(capi.hsx_grid(p), d)
+    _same(d.download(dtype), ref)
+
+
+def test_free_surface_with_layers_and_padding(oracle, capi):
+    for kw in (dict(nlayer=2), dict(add_pad=(3, 2, 1)), dict(offset=(1, 2, 0), mul_pad=(5, 5, 5)),
+               dict(auto_pad=True)):
+        p, o = _layouts(oracle, 3, (14, 12, 13), 6, np.float32, **kw)
+        u = _rand(np.random.default_rng(3), p.npoint, np.float32)
+        ref = u.copy()
+        oracle.free_surface(o, ref)
+        d = capi.DeviceArray.from_numpy(u)
+        capi.free_surface(capi.hsx_grid(p), d)
+        _same(d.download(np.float32), ref)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_fills_bit_exact(oracle, capi, prec):
+    dtype, _ = PRECS[prec]
+    p, o = _layouts(oracle, 3, (20, 17, 15), 8, dtype)
+    g = capi.hsx_grid(p)
+    base = np.zeros(p.npoint, dtype)
+    d = capi.DeviceArray.from_numpy(base)
+    for ptype in (oracle.INNER_POINTS, oracle.ALL_POINTS, oracle.I2HALO1, oracle.MIDDLE_POINT):
+        r = oracle.index_range(o, ptype)
+        ref = base.copy()
+        oracle.fill_const(o, ref, ptype, 0.7)
+        d.upload(base)
+        capi.fill_const(g, d, r, dtype(0.7))
+        _same(d.download(dtype), ref)
+    # separable sine fill (Grid::fill with FUNC_SINE on every axis)
+    orig = [oracle.unit_grid(20 if a == 0 else 17 if a == 1 else 15, o.ax[a].inner_start,
+                             fp64=dtype == np.float64)[1] for a in range(3)]
+    ref = base.copy()
+    tabs = oracle.fill_sep(o, ref, oracle.ALL_POINTS, (1, 2, 1), (2.1, 0.7, 3.3), -1.3, orig, p.d, 3)
+    d.upload(base)
+    capi.fill_separable(g, d, oracle.index_range(o, oracle.ALL_POINTS), *tabs, -1.3)
+    _same(d.download(dtype), ref)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_reductions(oracle, capi, prec):
+    dtype, _ = PRECS[prec]
+    p, o = _layouts(oracle, 3, (40, 33, 21), 8, dtype)
+    rng = np.random.default_rng(11)
+    u = _rand(rng, p.npoint, dtype)
+    v = _rand(rng, p.npoint, dtype)
+    v[::7] = dtype(1e-7)  # exercise the |ref| < MAX_ERR_FLOAT branch of maxErr
+    g = capi.hsx_grid(p)
+    du, dv = capi.DeviceArray.from_numpy(u), capi.DeviceArray.from_numpy(v)
+    maxerrf = 1e-11 if dtype == np.float64 else 1e-5
+    for ptype in (oracle.INNER_POINTS, oracle.ALL_POINTS):
+        r = oracle.index_range(o, ptype)
+        a = capi.sum_abs(g, du, r)
+        assert abs(a / oracle.reduce("sum_abs", o, u, ptype=ptype) - 1) < 1e-12
+        a = capi.sum_abs_diff(g, du, dv, r)
+        assert abs(a / oracle.reduce("sum_abs_diff", o, u, v, ptype=ptype) - 1) < 1e-12
+        s1, s2 = capi.l1_sums(g, du, dv, r)
+        assert abs(s1 / s2 / oracle.reduce("l1_err", o, u, v, ptype=ptype) - 1) < 1e-12
+        assert capi.max_err(g, du, dv, r, maxerrf) == oracle.reduce("max_err", o, u, v, ptype=ptype)
+        mn, mx = capi.min_max(g, du, r)
+        assert mn == oracle.reduce("min", o, u, ptype=ptype)
+        assert mx == oracle.reduce("max", o, u, ptype=ptype)
+
+
+def test_array_ops(capi):
+    p = make_layout(3, (16, 15, 14), 4)
+    g = capi.hsx_grid(p)
+    rng = np.random.default_rng(5)
+    u, v, w = (_rand(rng, p.npoint, np.float32) for _ in range(3))
+    du, dv, dw = (capi.DeviceArray.from_numpy(x) for x in (u, v, w))
+    capi.array_op(g, capi.HSX_ARRAY_ADD, dw, du, dv)
+    _same(dw.download(np.float32), u + v)
+    capi.array_op(g, capi.HSX_ARRAY_MULTIPLY, dw, du, dv)
+    _same(dw.download(np.float32), u * v)
+    capi.array_op(g, capi.HSX_ARRAY_ADDTO, dw, du)
+    _same(dw.download(np.float32), u * v + u)
+    capi.array_op(g, capi.HSX_ARRAY_COPY, dw, du)
+    _same(dw.download(np.float32), u)
+    capi.array_op(g, capi.HSX_ARRAY_FILL, dw, val=2.5)
+    _same(dw.download(np.float32), np.full_like(u, 2.5))
+
+
+@pytest.mark.parametrize("dim,n,order", [(3, (30, 28, 26), 8), (3, (20, 20, 20), 16), (2, (60, 50), 4),
+                                         (1, (90,), 12), (3, (64, 40, 33), 2)])
+@pytest.mark.parametrize("split", [False, True])
+def test_propagation_bit_exact(oracle, capi, family, dim, n, order, split):
+    """30 leapfrog steps (free surface + fused step, or Laplacian + update) on the
+    device give the oracle's wavefield bit for bit."""
+    print(f"propagation dim={dim} n={n} order={order} split={split} family={family}", flush=True)
+    p, o = _layouts(oracle, dim, n, order, np.float32)
+    rng = np.random.default_rng(order + dim)
+    prc = _rand(rng, p.npoint, np.float32)
+    prn = _rand(rng, p.npoint, np.float32)
+    coef = np.zeros(p.npoint, np.float32)
+    oracle.fill_const(o, coef, oracle.INNER_POINTS, 0.05)
+    lap = np.zeros(p.npoint, np.float32)
+    iv = oracle.inv2(p.d, np.float32)
+    g = capi.hsx_grid(p)
+    dprc, dprn, dcoef, dlap = (capi.DeviceArray.from_numpy(x) for x in (prc, prn, coef, lap))
+    a, b = prn.copy(), prc.copy()
+    for it in range(30):
+        oracle.free_surface(o, b)
+        capi.free_surface(g, dprc)
+        if split:
+            oracle.laplacian(o, b, lap, iv)
+            oracle.update_pressure(o, a, b, coef, lap)
+            capi.laplacian(g, dprc, dlap)
+            capi.update_pressure(g, dprn, dprc, dcoef, dlap)
+        else:
+            oracle.pressure_fd(o, a, b, coef, iv)
+            capi.ac2_step(g, dprn, dprc, dcoef)
+        a, b = b, a
+        dprn, dprc = dprc, dprn
+    _same(dprc.download(np.float32), b)
+    _same(dprn.download(np.float32), a)
+
+
+def test_invalid_arguments_are_rejected(capi):
+    """Host-side validation: no kernel is launched for a bad descriptor."""
+    p = make_layout(3, (10, 10, 10), 8)
+    g = capi.hsx_grid(p)
+    d = capi.DeviceArray(p.npoint * 4)
+    bad = capi.hsx_grid(p)
+    bad.fd_order = 9
+    with pytest.raises(capi.HsxError):
+        capi.ac2_step(bad, d, d, d)
+    bad = capi.hsx_grid(p)
+    bad.inner[1] = p.n1 - 1  # inner + halo would read past the row
+    with pytest.raises(capi.HsxError):
+        capi.laplacian(bad, d, d)
+    with pytest.raises(capi.HsxError):
+        capi.ac2_step(g, d, d, d)  # prn == prc
+    with pytest.raises(capi.HsxError):
+        capi.fd_d2(g, 3, d, d)
+    with pytest.raises(capi.HsxError):
+        capi.sum_abs(g, d, [0, p.n1, 0, 0, 0, 0])

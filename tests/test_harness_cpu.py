@@ -1,0 +1,141 @@
+"""Host logic that needs no GPU: the CPU build of the harness (oracle backends),
+the product binaries' command line, and the C ABI library's exported symbols."""
+import json
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import ROOT, report_value, run_bin
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+CPU = os.path.join(ROOT, "oracle", "_build", "hpcscan_cpu")
+CPU64 = os.path.join(ROOT, "oracle", "_build", "hpcscan_cpu_fp64")
+HEADER = os.path.join(ROOT, "include", "hpcscan_mi355x.h")
+
+
+def test_cpu_harness_reproduces_published_run(oracle, tmp_path):
+    """misc/fileForReadme/runPropaTestCase.txt: -testCase Propa -testMode CacheBlk."""
+    ref = json.load(open(os.path.join(GOLDEN, "propa_default_61.json")))
+    rc, out = run_bin(CPU, ["-testCase", "Propa", "-testMode", "CacheBlk", "-ntry", "2"], tmp_path)
+    assert rc == 0
+    for k, v in ref.items():
+        if k.startswith("_"):
+            continue
+        assert report_value(out, k) == v, (k, report_value(out, k), v)
+    assert " Test PASSED" in out and "FAILED" not in out
+
+
+def test_cpu_harness_perf_log_columns(oracle, tmp_path):
+    run_bin(CPU, ["-testCase", "Propa", "-testMode", "Baseline", "-ntry", "1", "-nt", "5"], tmp_path)
+    line = open(tmp_path / "hpcscan.perf.Propa.log").read().split()
+    assert line[1:4] == ["Propa", "Baseline", "Ac2Standard"]
+    assert len(line) == 4 + 9 + 14          # string columns, cols 1-9, cols 10-23
+    assert line[4 + 8] == "8"               # col 9 fdOrder
+    assert int(line[4 + 15]) == 5           # col 16 nt
+
+
+@pytest.mark.parametrize("mode", ["Baseline", "CacheBlk"])
+def test_cpu_validation_subset_zero_failed(oracle, tmp_path, mode):
+    """Stencil part of script/testDriver.sh on the CPU oracle: 0 FAILED."""
+    failed = passed = 0
+    for tc in ("FD_D2", "Propa"):
+        for dim in ("1", "2", "3"):
+            for order in ("2", "6", "8", "16"):
+                for extra in ([], ["-propagator", "Ac2SplitComp"]) if tc == "Propa" else ([],):
+                    rc, out = run_bin(CPU, ["-testCase", tc, "-testMode", mode, "-ntry", "1", "-dim", dim,
+                                            "-fdOrder", order] + extra, tmp_path)
+                    assert rc == 0
+                    passed += out.count("Test PASSED")
+                    failed += out.count("Test FAILED")
+    assert failed == 0 and passed > 0
+
+
+def test_split_and_standard_print_same_error(oracle, tmp_path):
+    outs = []
+    for prop in ("Ac2Standard", "Ac2SplitComp"):
+        rc, out = run_bin(CPU, ["-testCase", "Propa", "-testMode", "CacheBlk", "-ntry", "1", "-n1", "40", "-n2",
+                                "40", "-n3", "40", "-nt", "60", "-propagator", prop], tmp_path)
+        outs.append(report_value(out, "errTestCase2"))
+    assert outs[0] == outs[1]
+
+
+def test_cpu_harness_fp64(oracle, tmp_path):
+    rc, out = run_bin(CPU64, ["-testCase", "Propa", "-testMode", "Baseline", "-ntry", "1"], tmp_path)
+    assert rc == 0 and "FP64 [8 Bytes]" in out and " Test PASSED" in out
+
+
+def _product_bins():
+    from hpcscan_amd import binary
+
+    return [binary(p) for p in ("fp32", "fp64", "fp32c64")]
+
+
+def _have_gpu():
+    try:
+        from hpcscan_amd import capi
+
+        return capi.device_count() > 0
+    except Exception:
+        return False
+
+
+@pytest.mark.parametrize("path", _product_bins())
+def test_product_cli(tmp_path, path):
+    if not os.path.exists(path):
+        pytest.fail(f"{path} is not built: run __graft_entry__.build()")
+    rc, out = run_bin(path, ["-h"], tmp_path)
+    assert rc == 0
+    for flag in ("-testCase", "-testMode", "-fdOrder", "-propagator", "-hipKernel", "-gpuDevice", "-snapDt"):
+        assert flag in out
+    rc, out = run_bin(path, ["-v"], tmp_path)
+    assert "Mode MI355X\tENABLED" in out and "Mode Baseline" not in out   # no CPU backend in the product
+    rc, out = run_bin(path, ["-testCase", "Propa", "-fdOrder", "5"], tmp_path)
+    assert rc != 0 and "FD space order should be" in out
+    rc, out = run_bin(path, ["-testCase", "Propa", "-bogus"], tmp_path)
+    assert rc != 0 and "Unknown argument" in out
+    rc, out = run_bin(path, ["-testCase", "Propa", "-testMode", "Baseline"], tmp_path)
+    assert "not supported or invalid gridMode" in out
+
+
+def test_product_fails_loudly_without_gpu(tmp_path):
+    from hpcscan_amd import binary
+
+    if _have_gpu():
+        pytest.skip("a GPU is visible: the no-GPU error path cannot be exercised here")
+    rc, out = run_bin(binary(), ["-testCase", "Propa", "-testMode", "MI355X"], tmp_path)
+    assert "hsx_init failed" in out and "F A T A L" in out
+    assert "Test PASSED" not in out
+
+
+def test_c_abi_exports_every_declared_symbol():
+    """The library loads (after torch, sharing its HIP runtime) and exports every
+    entry point of include/hpcscan_mi355x.h; no compute call is made."""
+    from hpcscan_amd import capi
+
+    L = capi.lib()
+    text = open(HEADER).read()
+    declared = set(re.findall(r"\b(hsx_[a-z0-9_]+)\s*\(", text))
+    assert len(declared) >= 30
+    for name in declared:
+        assert hasattr(L, name), name
+    assert declared == set(capi.SIGNATURES), declared ^ set(capi.SIGNATURES)
+    assert L.hsx_abi_version() == 1
+    # symbols really exported by the shared object, checked with nm
+    nm = subprocess.run(["nm", "-D", "--defined-only", capi.LIB_PATH], capture_output=True, text=True).stdout
+    for name in declared:
+        assert re.search(rf"\bT {name}$", nm, re.M), name
+
+
+def test_c_abi_argument_errors_without_gpu():
+    """Descriptor validation happens on the host before any device work."""
+    from hpcscan_amd import capi
+    from hpcscan_amd.layout import make_layout
+
+    if _have_gpu():
+        pytest.skip("GPU visible: covered by the GPU tests")
+    g = capi.hsx_grid(make_layout(3, (10, 10, 10), 8))
+    rc = capi.lib().hsx_ac2_step(None, None, None, None, None)
+    assert rc != 0
+    assert capi.last_error()
