@@ -100,6 +100,14 @@ def gpu_leg(args):
         if ev:
             ev[1].record()
 
+    if args.calibrate_copy:
+        # one plain copy of a whole grid (npoint*4 bytes read + written): the
+        # reference kernel for scaling the FETCH_SIZE / WRITE_SIZE counters
+        scratch = capi.DeviceArray(nbytes)
+        capi.array_op(g, capi.HSX_ARRAY_COPY, scratch, prc)
+        capi.sync()
+        scratch.free()
+
     a, b = prn, prc
     for _ in range(args.warmup):
         step(a, b)
@@ -151,16 +159,25 @@ def cpu_leg(args):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
 
+    from hpcscan_amd.layout import make_layout
+
     n, order = args.n, args.order
     g = po.layout(3, [n, n, n], order)
+    p = make_layout(3, (n, n, n), order)
     d = [1.0 / (n - 1)] * 3
     iv = po.inv2(d, np.float32)
     prn = np.zeros(g.npoint, np.float32)
     prc = np.zeros(g.npoint, np.float32)
     coef = np.zeros(g.npoint, np.float32)
-    po.fill_const(g, coef, po.INNER_POINTS, 1e-7)
-    rng = np.random.default_rng(0)
-    prc[:] = rng.standard_normal(g.npoint, dtype=np.float32)
+    dt = np.float32(propa_scalars(n, order))
+    po.fill_const(g, coef, po.INNER_POINTS, float(np.float32(dt * dt)))
+    orig = [float(np.float32(-g.ax[a].inner_start * d[a])) for a in range(3)]
+    gamma = math.sqrt(12)
+    po.fill_sep(g, prn, po.ALL_POINTS, (1, 1, 1), [PI_REF * 2] * 3,
+                math.sin(PI_REF * float(np.float32(-2.0 * dt)) * gamma), orig, d, 3)
+    po.fill_sep(g, prc, po.ALL_POINTS, (1, 1, 1), [PI_REF * 2] * 3,
+                math.sin(PI_REF * float(np.float32(-dt)) * gamma), orig, d, 3)
+    del p
     s = po.sched(1, (9999, 4, 16))
     steps, t_used = 0, 0.0
     while (t_used < args.cpu_seconds or steps < 2) and steps < 1000:
@@ -188,6 +205,8 @@ def main():
     ap.add_argument("--kernel", default="auto", choices=["auto", "ref", "tiled"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run one full-grid copy kernel first (counter calibration for profiles)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))

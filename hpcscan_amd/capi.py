@@ -78,6 +78,7 @@ SIGNATURES = {
     "hsx_event_elapsed_ms": (_i, [_vp, _vp, _P(ctypes.c_float)]),
     "hsx_set_kernel": (_i, [_i]),
     "hsx_get_kernel": (_i, []),
+    "hsx_set_tile": (_i, [_i, _i]),
     "hsx_fill_const": (_i, [_G, _vp, _R, _d, _vp]),
     "hsx_fill_separable": (_i, [_G, _vp, _R, _P(_d), _P(_d), _P(_d), _d, _vp]),
     "hsx_array_op": (_i, [_G, _i, _vp, _vp, _vp, _d, _vp]),
@@ -139,6 +140,10 @@ def device_info(device: int = 0) -> dict:
 
 def set_kernel(family: int):
     check(lib().hsx_set_kernel(family))
+
+
+def set_tile(lanes_n1: int = 0, chunk_planes: int = 0):
+    check(lib().hsx_set_tile(lanes_n1, chunk_planes))
 
 
 def hsx_grid(g: GridLayout, compute_fp64: Optional[bool] = None) -> HsxGrid:

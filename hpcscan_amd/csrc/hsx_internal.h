@@ -116,7 +116,14 @@ inline int dispatch_dim(int dim, F &&f)
 // grid does not fit their assumptions so the caller uses the reference kernels
 int tiled_ac2_step(const hsx_grid *g, void *prn, const void *prc, const void *coef, hipStream_t s);
 int tiled_laplacian(const hsx_grid *g, const void *u, void *w, hipStream_t s);
-int tiled_fd_d2(const hsx_grid *g, int axis, const void *u, void *w, hipStream_t s);
+int tiled_fd_d2_x(const hsx_grid *g, const void *u, void *w, hipStream_t s);
+int tiled_fd_d2_y(const hsx_grid *g, const void *u, void *w, hipStream_t s);
+int tiled_fd_d2_z(const hsx_grid *g, const void *u, void *w, hipStream_t s);
+inline int tiled_fd_d2(const hsx_grid *g, int axis, const void *u, void *w, hipStream_t s)
+{
+    return axis == 0 ? tiled_fd_d2_x(g, u, w, s) : axis == 1 ? tiled_fd_d2_y(g, u, w, s) : tiled_fd_d2_z(g, u, w, s);
+}
+int set_tile(int lx, int chunk);
 
 } // namespace hsx
 

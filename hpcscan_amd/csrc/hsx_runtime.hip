@@ -315,4 +315,6 @@ int hsx_set_kernel(int family)
 
 int hsx_get_kernel(void) { return g.family; }
 
+int hsx_set_tile(int lanes_n1, int chunk_planes) { return set_tile(lanes_n1, chunk_planes); }
+
 } // extern "C"

@@ -1,0 +1,383 @@
+// LDS-tiled, n3-streaming stencil kernels for gfx950 (3D grids).
+//
+// One workgroup owns a tile of LX*VW points along n1 by BY rows along n2 and
+// marches along n3 over a chunk of planes:
+//  * n3 neighbours come from a per-thread register queue of the 2R+1 planes
+//    z-R..z+R (each slot holds VW consecutive n1 values);
+//  * n1 / n2 neighbours come from the current plane staged in LDS with its
+//    +-R halo (double buffered, one barrier per plane);
+//  * loads and stores are VW-wide (16 bytes per lane: float4 / double2) and a
+//    wave covers contiguous n1, so every global access is coalesced;
+//  * the next plane (queue front, halo rows/columns, prn and coef) is loaded
+//    while the current one is computed.
+// The queue is never shifted: the plane loop is unrolled by 2R+1 so that the
+// slot of plane z+k is a compile-time index.
+//
+// Per-point arithmetic is d2_values / leapfrog of hsx_stencil_point.h, i.e.
+// the reference operation order; results equal the reference kernels bit for
+// bit.  Launch: 1D grid of tiles x chunks, block ids remapped so that the
+// tiles handled by one XCD (one L2) are neighbours and share their halos.
+#ifndef HSX_TILED_H
+#define HSX_TILED_H
+
+#include "hsx_internal.h"
+#include "hsx_stencil_point.h"
+
+namespace hsx {
+
+enum TiledMode { MODE_AC2 = 0, MODE_LAP = 1, MODE_D2X = 2, MODE_D2Y = 3, MODE_D2Z = 4 };
+
+template <typename T, int VW>
+struct alignas(sizeof(T) * VW) Pack {
+    T v[VW];
+};
+
+template <typename TC>
+struct TiledArgs {
+    void *out;                // prn (MODE_AC2) or w
+    const void *u;            // prc / u
+    const void *coef;         // MODE_AC2 only
+    int64_t n1, n2, n12;      // row length, rows per plane, plane stride
+    int64_t s1, e1, s2, e2, s3, e3;
+    int64_t t1s;              // n1 origin of the first tile (multiple of VW)
+    int ntx, nty, nchunk, chunk;
+    Coefs<TC> c;
+};
+
+constexpr int ceil_div(int a, int b) { return (a + b - 1) / b; }
+constexpr int round_up(int a, int b) { return ceil_div(a, b) * b; }
+
+template <typename TS, typename TC, int R, int VW, int LX, int BY, int MODE>
+struct Tiled {
+    static constexpr int P = 2 * R + 1;               // queue length
+    static constexpr int HL = round_up(R, VW);        // halo slots left / right (multiple of VW)
+    static constexpr int W = HL + LX * VW + HL;       // LDS row pitch in elements
+    static constexpr int ROWS = BY + 2 * R;
+    static constexpr int PLANE = ROWS * W;            // elements of one LDS plane
+    static constexpr int NHG = HL / VW;               // halo groups per side
+    static constexpr int NHR = ceil_div(2 * R, BY);   // n2 halo rows loaded per thread
+    static constexpr bool NEED_X = MODE == MODE_AC2 || MODE == MODE_LAP || MODE == MODE_D2X;
+    static constexpr bool NEED_Y = MODE == MODE_AC2 || MODE == MODE_LAP || MODE == MODE_D2Y;
+    static constexpr bool NEED_Z = MODE == MODE_AC2 || MODE == MODE_LAP || MODE == MODE_D2Z;
+    static constexpr bool NEED_LDS = NEED_X || NEED_Y;
+    using V = Pack<TS, VW>;
+    static size_t lds_bytes() { return NEED_LDS ? 2 * PLANE * sizeof(TS) : 0; }
+};
+
+template <typename TS, int VW>
+__device__ __forceinline__ Pack<TS, VW> vload(const TS *p)
+{
+    return *reinterpret_cast<const Pack<TS, VW> *>(p);
+}
+
+template <typename TS, int VW>
+__device__ __forceinline__ Pack<TS, VW> vzero()
+{
+    Pack<TS, VW> z;
+#pragma unroll
+    for (int v = 0; v < VW; v++) z.v[v] = (TS)0;
+    return z;
+}
+
+template <typename TS, typename TC, int R, int VW, int LX, int BY, int MODE>
+__global__ void __launch_bounds__(LX *BY) k_tiled(TiledArgs<TC> a)
+{
+    using K = Tiled<TS, TC, R, VW, LX, BY, MODE>;
+    using V = typename K::V;
+    constexpr int P = K::P, HL = K::HL, W = K::W, NHG = K::NHG, NHR = K::NHR;
+    extern __shared__ __align__(16) unsigned char hsx_lds_raw[];
+    TS *lds = reinterpret_cast<TS *>(hsx_lds_raw);
+
+    // ---- XCD-aware block -> (chunk, tile) map: consecutive logical blocks share an XCD
+    const int nblk = a.ntx * a.nty * a.nchunk;
+    const int per_xcd = (nblk + 7) / 8;
+    const int logical = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (logical >= nblk) return;                       // whole block leaves, before any barrier
+    const int tile = logical % (a.ntx * a.nty);
+    const int chunk = logical / (a.ntx * a.nty);
+    const int tx = threadIdx.x, ty = threadIdx.y;
+
+    const int64_t i1 = a.t1s + (int64_t)(tile % a.ntx) * (LX * VW) + tx * VW;   // first n1 index of this lane
+    const int64_t t2s = a.s2 + (int64_t)(tile / a.ntx) * BY;                   // first n2 row of the tile
+    const int64_t i2 = t2s + ty;
+    const int64_t z0 = a.s3 + (int64_t)chunk * a.chunk;
+    const int64_t z1 = min(z0 + a.chunk, a.e3 + 1);    // planes [z0, z1)
+
+    const bool lane_in_row = i1 + VW - 1 < a.n1;        // vector fully inside the row
+    const bool row_ok = i2 < a.n2;
+    const TS *u = reinterpret_cast<const TS *>(a.u);
+    const int64_t n1 = a.n1, n12 = a.n12;
+
+    // ---- queue: slot (z - z0 + R) % P holds plane z
+    V q[P];
+#pragma unroll
+    for (int k = 0; k < P; k++) q[k] = vzero<TS, VW>();
+    const bool col_ok = lane_in_row && row_ok;
+    const int64_t col = i1 + i2 * n1;
+#pragma unroll
+    for (int k = 0; k < 2 * R; k++) {      // planes z0-R .. z0+R-1
+        const int64_t z = z0 - R + k;
+        if (col_ok) q[k] = vload<TS, VW>(u + col + z * n12);
+    }
+
+    // prefetch registers for the first plane
+    V front = vzero<TS, VW>();
+    if (col_ok && K::NEED_Z) front = vload<TS, VW>(u + col + (z0 + R) * n12);
+    if (col_ok && !K::NEED_Z) q[R] = vload<TS, VW>(u + col + z0 * n12);
+    V hrow[NHR];          // n2 halo rows of the next plane
+    V hcol = vzero<TS, VW>();   // n1 halo group of the next plane (edge lanes)
+    V nprn = vzero<TS, VW>(), ncoef = vzero<TS, VW>();
+
+    // which halo loads this thread owns
+    int64_t hrow_i2[NHR];
+    int hrow_lds[NHR];
+    bool hrow_ok[NHR];
+#pragma unroll
+    for (int k = 0; k < NHR; k++) {
+        const int h = ty + k * BY;                         // 0 .. 2R-1 for the halo rows
+        const int row = h < R ? h : BY + h;                // LDS row
+        hrow_lds[k] = row;
+        hrow_i2[k] = t2s + row - R;
+        hrow_ok[k] = K::NEED_Y && h < 2 * R && hrow_i2[k] < a.n2 && lane_in_row;
+        hrow[k] = vzero<TS, VW>();
+    }
+    const bool left = K::NEED_X && tx < NHG;
+    const bool right = K::NEED_X && tx >= LX - NHG;
+    const int64_t hc_i1 = left ? a.t1s + (int64_t)(tile % a.ntx) * (LX * VW) - HL + tx * VW
+                               : a.t1s + (int64_t)(tile % a.ntx) * (LX * VW) + LX * VW + (tx - (LX - NHG)) * VW;
+    const int hc_lds = left ? tx * VW : HL + LX * VW + (tx - (LX - NHG)) * VW;
+    const bool hc_ok = (left || right) && row_ok && hc_i1 >= 0 && hc_i1 + VW - 1 < a.n1;
+
+    auto load_plane_side = [&](int64_t z) {
+#pragma unroll
+        for (int k = 0; k < NHR; k++)
+            if (hrow_ok[k]) hrow[k] = vload<TS, VW>(u + i1 + hrow_i2[k] * n1 + z * n12);
+        if (hc_ok) hcol = vload<TS, VW>(u + hc_i1 + i2 * n1 + z * n12);
+        if (MODE == MODE_AC2 && col_ok) {
+            nprn = vload<TS, VW>(reinterpret_cast<const TS *>(a.out) + col + z * n12);
+            ncoef = vload<TS, VW>(reinterpret_cast<const TS *>(a.coef) + col + z * n12);
+        }
+    };
+    load_plane_side(z0);
+
+    // compute mask: which of the VW points of this lane are inner points
+    bool inner[VW];
+#pragma unroll
+    for (int v = 0; v < VW; v++) inner[v] = i1 + v >= a.s1 && i1 + v <= a.e1 && i2 <= a.e2;
+    bool any_inner = false;
+#pragma unroll
+    for (int v = 0; v < VW; v++) any_inner |= inner[v];
+
+    const TC *A = a.c.A;
+    int buf = 0;
+    for (int64_t zb = z0; zb < z1; zb += P) {
+#pragma unroll
+        for (int s = 0; s < P; s++) {
+            const int64_t z = zb + s;
+            if (z >= z1) break;
+            // queue slots are compile-time after unrolling: plane z sits in (s + R) % P
+            const int cz = (s + R) % P;
+            // the front loaded during the previous plane is plane z+R: store it
+            if (K::NEED_Z) q[(s + 2 * R) % P] = front;
+            const V cur = q[cz];
+            const V prn_v = nprn, coef_v = ncoef;
+            V hrow_cur[NHR];
+#pragma unroll
+            for (int k = 0; k < NHR; k++) hrow_cur[k] = hrow[k];
+            const V hcol_cur = hcol;
+
+            // stage plane z in LDS
+            TS *pl = lds + buf * K::PLANE;
+            if (K::NEED_LDS) {
+                *reinterpret_cast<V *>(pl + (ty + R) * W + HL + tx * VW) = cur;
+#pragma unroll
+                for (int k = 0; k < NHR; k++)
+                    if (hrow_ok[k]) *reinterpret_cast<V *>(pl + hrow_lds[k] * W + HL + tx * VW) = hrow_cur[k];
+                if (hc_ok) *reinterpret_cast<V *>(pl + (ty + R) * W + hc_lds) = hcol_cur;
+            }
+
+            // prefetch plane z+1 (queue front z+1+R, halos, prn, coef)
+            if (z + 1 < z1) {
+                if (K::NEED_Z && col_ok) front = vload<TS, VW>(u + col + (z + 1 + R) * n12);
+                if (!K::NEED_Z && col_ok) q[(s + 1 + R) % P] = vload<TS, VW>(u + col + (z + 1) * n12);
+                load_plane_side(z + 1);
+            }
+
+            if (K::NEED_LDS) __syncthreads();
+
+            if (any_inner) {
+                // n1 neighbours: the row segment [HL + tx*VW - HL, HL + tx*VW + VW + HL)
+                TS xr[2 * HL + VW];
+                TS yu[R][VW], yd[R][VW];
+                if (K::NEED_X) {
+                    const TS *rowp = pl + (ty + R) * W + tx * VW;
+#pragma unroll
+                    for (int g = 0; g < 2 * NHG + 1; g++) {
+                        const V t = g == NHG ? cur : *reinterpret_cast<const V *>(rowp + g * VW);
+#pragma unroll
+                        for (int v = 0; v < VW; v++) xr[g * VW + v] = t.v[v];
+                    }
+                }
+                if (K::NEED_Y) {
+#pragma unroll
+                    for (int j = 1; j <= R; j++) {
+                        const V tu = *reinterpret_cast<const V *>(pl + (ty + R + j) * W + HL + tx * VW);
+                        const V td = *reinterpret_cast<const V *>(pl + (ty + R - j) * W + HL + tx * VW);
+#pragma unroll
+                        for (int v = 0; v < VW; v++) {
+                            yu[j - 1][v] = tu.v[v];
+                            yd[j - 1][v] = td.v[v];
+                        }
+                    }
+                }
+                V res;
+#pragma unroll
+                for (int v = 0; v < VW; v++) {
+                    const TS c0 = cur.v[v];
+                    TS p[R], m[R];
+                    TC d1 = 0, d2 = 0, d3 = 0;
+                    if (K::NEED_X) {
+#pragma unroll
+                        for (int j = 1; j <= R; j++) {
+                            p[j - 1] = xr[HL + v + j];
+                            m[j - 1] = xr[HL + v - j];
+                        }
+                        d1 = d2_values<TS, TC, R>(c0, p, m, A, a.c.inv2[0]);
+                    }
+                    if (K::NEED_Y) {
+#pragma unroll
+                        for (int j = 1; j <= R; j++) {
+                            p[j - 1] = yu[j - 1][v];
+                            m[j - 1] = yd[j - 1][v];
+                        }
+                        d2 = d2_values<TS, TC, R>(c0, p, m, A, a.c.inv2[1]);
+                    }
+                    if (K::NEED_Z) {
+#pragma unroll
+                        for (int j = 1; j <= R; j++) {
+                            p[j - 1] = q[(s + R + j) % P].v[v];
+                            m[j - 1] = q[(s + R - j + P) % P].v[v];
+                        }
+                        d3 = d2_values<TS, TC, R>(c0, p, m, A, a.c.inv2[2]);
+                    }
+                    TS o;
+                    if (MODE == MODE_AC2) {
+                        const TC lap = d1 + d2 + d3;
+                        o = leapfrog<TS, TC>(c0, prn_v.v[v], coef_v.v[v], lap);
+                    } else if (MODE == MODE_LAP) {
+                        o = (TS)(d1 + d2 + d3);
+                    } else if (MODE == MODE_D2X) {
+                        o = (TS)d1;
+                    } else if (MODE == MODE_D2Y) {
+                        o = (TS)d2;
+                    } else {
+                        o = (TS)d3;
+                    }
+                    res.v[v] = o;
+                }
+                TS *outp = reinterpret_cast<TS *>(a.out) + col + z * n12;
+                bool all = true;
+#pragma unroll
+                for (int v = 0; v < VW; v++) all &= inner[v];
+                if (all) {
+                    *reinterpret_cast<V *>(outp) = res;
+                } else {
+#pragma unroll
+                    for (int v = 0; v < VW; v++)
+                        if (inner[v]) outp[v] = res.v[v];
+                }
+            }
+            buf ^= 1;
+        }
+    }
+}
+
+// ---- launch configuration -------------------------------------------------------------------
+// Tile shape: LX lanes along n1 (64 = one wave per row, 16/32 for narrow grids),
+// BY = 8 rows along n2, chunk = planes per workgroup (0 = sized to give the
+// 256 CUs >= 2048 workgroups).  hsx_set_tile() overrides LX and the chunk.
+extern int g_tile_lx, g_tile_chunk;   // defined in hsx_tiled_ac2.hip
+constexpr int TILE_BY = 8;
+
+inline int pick_lx(const hsx_grid *g, int vw)
+{
+    if (g_tile_lx) return g_tile_lx;
+    const int64_t N1 = g->inner[1] - g->inner[0] + 1;
+    return N1 <= 16 * vw ? 16 : N1 <= 32 * vw ? 32 : 64;
+}
+
+template <typename TS, typename TC, int R, int VW, int LX, int BY, int MODE>
+inline int launch_tiled(const hsx_grid *g, void *out, const void *u, const void *coef, int chunk, hipStream_t s)
+{
+    using K = Tiled<TS, TC, R, VW, LX, BY, MODE>;
+    TiledArgs<TC> a;
+    a.out = out;
+    a.u = u;
+    a.coef = coef;
+    a.n1 = g->n1;
+    a.n2 = g->n2;
+    a.n12 = g->n1 * g->n2;
+    a.s1 = g->inner[0];
+    a.e1 = g->inner[1];
+    a.s2 = g->inner[2];
+    a.e2 = g->inner[3];
+    a.s3 = g->inner[4];
+    a.e3 = g->inner[5];
+    a.t1s = (a.s1 / VW) * VW;
+    const int64_t span1 = a.e1 - a.t1s + 1;
+    a.ntx = (int)((span1 + LX * VW - 1) / (LX * VW));
+    a.nty = (int)((a.e2 - a.s2 + 1 + BY - 1) / BY);
+    const int64_t N3 = a.e3 - a.s3 + 1;
+    const int64_t tiles = (int64_t)a.ntx * a.nty;
+    // enough workgroups to fill 256 CUs a few times, chunks no shorter than 4R planes
+    int64_t nch = chunk > 0 ? (N3 + chunk - 1) / chunk : (2048 + tiles - 1) / tiles;
+    if (nch < 1) nch = 1;
+    int64_t len = (N3 + nch - 1) / nch;
+    if (chunk <= 0 && len < 4 * R) len = N3 < 4 * R ? N3 : 4 * R;
+    a.chunk = (int)len;
+    a.nchunk = (int)((N3 + len - 1) / len);
+    a.c = make_coefs<TC>(g);
+    const int64_t nblk = tiles * a.nchunk;
+    if (nblk > (int64_t)1 << 30) return fail(HSX_ERR_UNSUPPORTED, "too many tiles");
+    const unsigned grid = (unsigned)(((nblk + 7) / 8) * 8);
+    static_assert(2 * K::PLANE * sizeof(TS) <= 65536 || !K::NEED_LDS, "LDS tile above 64 KiB");
+    hipLaunchKernelGGL((k_tiled<TS, TC, R, VW, LX, BY, MODE>), dim3(grid), dim3(LX, BY), K::lds_bytes(), s, a);
+    HSX_LAUNCHED("k_tiled");
+    return HSX_OK;
+}
+
+// alignment and size conditions of the tiled kernels
+inline int tiled_fits(const hsx_grid *g, int vw_elems, size_t elem)
+{
+    if (g->dim != 3) return fail(HSX_ERR_UNSUPPORTED, "tiled kernels need a 3D grid");
+    if (g->n1 % vw_elems) return fail(HSX_ERR_UNSUPPORTED, "n1 %% %d != 0", vw_elems);
+    (void)elem;
+    return HSX_OK;
+}
+
+template <int MODE>
+inline int run_tiled(const hsx_grid *g, void *out, const void *u, const void *coef, hipStream_t s)
+{
+    const int R = g->fd_order / 2;
+    return dispatch_types(g, [&](auto ts, auto tc) -> int {
+        using TS = decltype(ts);
+        using TC = decltype(tc);
+        constexpr int VW = 16 / sizeof(TS);
+        int rc = tiled_fits(g, VW, sizeof(TS));
+        if (rc) return rc;
+        if (((uintptr_t)out | (uintptr_t)u | (uintptr_t)coef) % 16)
+            return fail(HSX_ERR_UNSUPPORTED, "buffers not 16-byte aligned");
+        const int lx = pick_lx(g, VW);
+        return dispatch_R(R, [&](auto rc_) -> int {
+            constexpr int RR = decltype(rc_)::value;
+            constexpr int BY = TILE_BY;
+            if (lx == 16) return launch_tiled<TS, TC, RR, VW, 16, BY, MODE>(g, out, u, coef, g_tile_chunk, s);
+            if (lx == 32) return launch_tiled<TS, TC, RR, VW, 32, BY, MODE>(g, out, u, coef, g_tile_chunk, s);
+            return launch_tiled<TS, TC, RR, VW, 64, BY, MODE>(g, out, u, coef, g_tile_chunk, s);
+        });
+    });
+}
+
+} // namespace hsx
+
+#endif

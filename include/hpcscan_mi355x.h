@@ -95,6 +95,9 @@ int         hsx_event_record(void *event, void *stream);
 int         hsx_event_elapsed_ms(void *start, void *stop, float *ms);
 int         hsx_set_kernel(int family);
 int         hsx_get_kernel(void);
+/* tile shape of the tiled kernels: lanes along n1 (16, 32, 64; 0 = by grid
+ * width) and planes per workgroup along n3 (0 = sized to fill the GPU) */
+int         hsx_set_tile(int lanes_n1, int chunk_planes);
 
 /* ---- grid filling: Grid::fill (src/grid.cpp:1650-1826) -------------------- */
 /* u[range] = val (converted to the storage type) */

@@ -249,3 +249,77 @@ def test_invalid_arguments_are_rejected(capi):
         capi.fd_d2(g, 3, d, d)
     with pytest.raises(capi.HsxError):
         capi.sum_abs(g, d, [0, p.n1, 0, 0, 0, 0])
+
+
+def _n1_aligned(target, order, dtype, vw):
+    """smallest inner n1 >= target whose padded row length is a multiple of vw"""
+    for n in range(target, target + 16):
+        if make_layout(3, (n, 8, 8), order, fp64=dtype == np.float64).n1 % vw == 0:
+            return n
+    raise AssertionError("no aligned size")
+
+
+TILED_CASES = [
+    # (order, prec, n1 target, n2, n3, lanes, chunk)
+    (8, "f32", 72, 33, 29, 16, 5),      # several tiles in n1 and n2, chunk < 2R+1
+    (8, "f32", 72, 33, 29, 16, 9),      # chunk == 2R+1
+    (8, "f32", 130, 17, 40, 32, 13),
+    (8, "f32", 200, 24, 31, 64, 0),     # one wide tile, auto chunk
+    (2, "f32", 69, 20, 22, 16, 7),      # odd halo: n1 offset, tile origin below the inner start
+    (6, "f32", 69, 19, 23, 16, 4),
+    (16, "f32", 68, 25, 37, 16, 6),     # R = 8: two n2 halo rows per thread
+    (16, "f32", 140, 9, 20, 64, 0),
+    (4, "f64", 70, 21, 27, 16, 5),      # VW = 2
+    (12, "f64", 66, 30, 26, 32, 11),
+    (8, "f32c64", 72, 18, 21, 16, 5),   # FP32 storage, FP64 compute
+]
+
+
+@pytest.mark.parametrize("order,prec,n1,n2,n3,lanes,chunk", TILED_CASES)
+def test_tiled_shapes_bit_exact(oracle, capi, order, prec, n1, n2, n3, lanes, chunk):
+    """The LDS-tiled kernels, forced, on partial tiles and chunk boundaries."""
+    dtype, c64 = PRECS[prec]
+    vw = 16 // np.dtype(dtype).itemsize
+    n = (_n1_aligned(n1, order, dtype, vw), n2, n3)
+    print(f"tiled order={order} prec={prec} n={n} lanes={lanes} chunk={chunk}", flush=True)
+    p, o = _layouts(oracle, 3, n, order, dtype)
+    assert p.n1 % vw == 0
+    rng = np.random.default_rng(order * 7 + n3)
+    g = capi.hsx_grid(p, compute_fp64=c64)
+    u, coef, prn0, w0 = (_rand(rng, p.npoint, dtype) for _ in range(4))
+    iv = oracle.inv2(p.d, dtype, compute64=c64)
+    du, dc, dp, dw = (capi.DeviceArray.from_numpy(x) for x in (u, coef, prn0, w0))
+    capi.set_kernel(capi.HSX_KERNEL_TILED)
+    capi.set_tile(lanes, chunk)
+    try:
+        for axis in range(3):
+            ref = w0.copy()
+            oracle.fd_d2(o, axis, u, ref, iv, compute64=c64)
+            dw.upload(w0)
+            capi.fd_d2(g, axis, du, dw)
+            _same(dw.download(dtype), ref)
+        ref = w0.copy()
+        oracle.laplacian(o, u, ref, iv, compute64=c64)
+        dw.upload(w0)
+        capi.laplacian(g, du, dw)
+        _same(dw.download(dtype), ref)
+        ref = prn0.copy()
+        oracle.pressure_fd(o, ref, u, coef, iv, compute64=c64)
+        capi.ac2_step(g, dp, du, dc)
+        _same(dp.download(dtype), ref)
+    finally:
+        capi.set_tile(0, 0)
+        capi.set_kernel(capi.HSX_KERNEL_AUTO)
+
+
+def test_tiled_rejects_misaligned_rows(capi):
+    """Forced tiled kernels refuse rows that are not a multiple of 16 bytes."""
+    p = make_layout(3, (21, 18, 17), 8)   # n1 = 29
+    g = capi.hsx_grid(p)
+    d = capi.DeviceArray(p.npoint * 4)
+    capi.set_kernel(capi.HSX_KERNEL_TILED)
+    try:
+        with pytest.raises(capi.HsxError):
+            capi.laplacian(g, d, capi.DeviceArray(p.npoint * 4))
+    finally:
+        capi.set_kernel(capi.HSX_KERNEL_AUTO)
