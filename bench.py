@@ -70,6 +70,8 @@ def gpu_leg(args):
     capi.init(0)
     capi.set_kernel({"auto": capi.HSX_KERNEL_AUTO, "ref": capi.HSX_KERNEL_REF,
                      "tiled": capi.HSX_KERNEL_TILED}[args.kernel])
+    capi.set_tile(args.tile_lx, args.tile_chunk, args.tile_rows,
+                  {"default": 0, "cached": capi.HSX_STREAMS_CACHED, "nt": capi.HSX_STREAMS_NT}[args.streams])
     info = capi.device_info(0)
     n, order = args.n, args.order
     p = make_layout(3, (n, n, n), order)
@@ -136,7 +138,8 @@ def gpu_leg(args):
     alg_bytes = 16 * npt
     achieved = alg_bytes / avg_kernel_s / 1e9
     traffic = None
-    if os.path.exists(TRAFFIC_FILE) and (n, order, args.kernel) == (1000, 8, "auto"):
+    default_shape = (args.tile_lx, args.tile_rows, args.tile_chunk, args.streams) == (0, 0, 0, "default")
+    if os.path.exists(TRAFFIC_FILE) and (n, order, args.kernel) == (1000, 8, "auto") and default_shape:
         try:
             traffic = json.load(open(TRAFFIC_FILE)).get("hbm_bytes_per_launch")
         except Exception:
@@ -205,6 +208,14 @@ def main():
     ap.add_argument("--kernel", default="auto", choices=["auto", "ref", "tiled"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tile-lx", type=int, default=0, choices=[0, 16, 32, 64],
+                    help="lanes along n1 per tile of the fused kernel (0 = chosen from the grid width)")
+    ap.add_argument("--tile-rows", type=int, default=0, choices=[0, 8, 16],
+                    help="rows along n2 per tile (0 = library choice)")
+    ap.add_argument("--tile-chunk", type=int, default=0,
+                    help="n3 planes streamed per workgroup (0 = library choice)")
+    ap.add_argument("--streams", default="default", choices=["default", "cached", "nt"],
+                    help="cached or nontemporal prn/coef/output accesses of the fused kernel")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one full-grid copy kernel first (counter calibration for profiles)")
     args = ap.parse_args()
@@ -237,7 +248,9 @@ def main():
         "config": {"workload": f"Propa Ac2Standard FP32 FD order {args.order}, {args.n}^3 inner grid, "
                                f"eigen mode (2,2,2), free surface on all faces, dt at the CFL limit",
                    "grid": [args.n] * 3, "fd_order": args.order, "propagator": "Ac2Standard",
-                   "inner_points": npt, "kernel": args.kernel},
+                   "inner_points": npt, "kernel": args.kernel,
+                   "tile_lx": args.tile_lx, "tile_rows": args.tile_rows, "tile_chunk": args.tile_chunk,
+                   "streams": args.streams},
         "roofline": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in gpu["roofline"].items()},
         "cpu_baseline": cpu,
         "extra": {"fused_kernel_ms": round(gpu["kernel_ms"], 4),

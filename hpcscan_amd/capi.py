@@ -23,6 +23,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libhpcscan_mi355x.so")
 HSX_OK = 0
 HSX_FP32, HSX_FP64 = 0, 1
 HSX_KERNEL_AUTO, HSX_KERNEL_REF, HSX_KERNEL_TILED = 0, 1, 2
+HSX_STREAMS_CACHED, HSX_STREAMS_NT = 1, 2
 HSX_ARRAY_FILL, HSX_ARRAY_COPY, HSX_ARRAY_ADD, HSX_ARRAY_MULTIPLY, HSX_ARRAY_ADDTO = range(5)
 ALL_FACES = 0x3F
 
@@ -78,7 +79,7 @@ SIGNATURES = {
     "hsx_event_elapsed_ms": (_i, [_vp, _vp, _P(ctypes.c_float)]),
     "hsx_set_kernel": (_i, [_i]),
     "hsx_get_kernel": (_i, []),
-    "hsx_set_tile": (_i, [_i, _i]),
+    "hsx_set_tile": (_i, [_i, _i, _i, _i]),
     "hsx_fill_const": (_i, [_G, _vp, _R, _d, _vp]),
     "hsx_fill_separable": (_i, [_G, _vp, _R, _P(_d), _P(_d), _P(_d), _d, _vp]),
     "hsx_array_op": (_i, [_G, _i, _vp, _vp, _vp, _d, _vp]),
@@ -135,15 +136,17 @@ def device_info(device: int = 0) -> dict:
     p = HsxDeviceProps()
     check(lib().hsx_device_info(device, ctypes.byref(p)))
     return {"name": p.name.decode(), "arch": p.arch.decode(), "compute_units": p.compute_units,
-            "clock_mhz": p.clock_mhz, "hbm_bytes": p.hbm_bytes, "l2_bytes": p.l2_bytes}
+            "clock_mhz": p.clock_mhz, "hbm_bytes": p.hbm_bytes, "l2_bytes": p.l2_bytes,
+            "lds_bytes_per_block": p.lds_bytes_per_block}
 
 
 def set_kernel(family: int):
     check(lib().hsx_set_kernel(family))
 
 
-def set_tile(lanes_n1: int = 0, chunk_planes: int = 0):
-    check(lib().hsx_set_tile(lanes_n1, chunk_planes))
+def set_tile(lanes_n1: int = 0, chunk_planes: int = 0, rows_n2: int = 0, streams: int = 0):
+    """Tile shape of the tiled kernels; 0 = the library's choice for each."""
+    check(lib().hsx_set_tile(lanes_n1, rows_n2, chunk_planes, streams))
 
 
 def hsx_grid(g: GridLayout, compute_fp64: Optional[bool] = None) -> HsxGrid:

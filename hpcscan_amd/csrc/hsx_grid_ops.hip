@@ -4,6 +4,7 @@
 //
 // These are HBM-streaming passes: one thread per value, rows of the index box
 // mapped to waves so that n1 (contiguous) runs across the 64 lanes.
+#include <algorithm>
 #include <cfloat>
 #include <mutex>
 
@@ -56,17 +57,51 @@ __global__ void __launch_bounds__(256) k_fill_sep(TS *u, Box b, int64_t n1, int6
 
 // ---- array operations (src/grid.cpp:1830-1920) --------------------------------
 template <typename TS>
+__device__ __forceinline__ void array_elem(int op, TS *w, const TS *u, const TS *v, TS val, int64_t i)
+{
+    switch (op) {
+    case HSX_ARRAY_FILL: w[i] = val; break;
+    case HSX_ARRAY_COPY: w[i] = u[i]; break;
+    case HSX_ARRAY_ADD: w[i] = u[i] + v[i]; break;
+    case HSX_ARRAY_MULTIPLY: w[i] = u[i] * v[i]; break;
+    default: w[i] = w[i] + u[i]; break;
+    }
+}
+
+// 16-byte vectors per lane when every pointer is 16-byte aligned (the usual
+// case: whole grids from hsx_malloc), element loop for the tail
+template <typename TS>
+__global__ void __launch_bounds__(256) k_array16(int op, TS *w, const TS *u, const TS *v, TS val, int64_t n)
+{
+    constexpr int VW = 16 / sizeof(TS);
+    struct alignas(16) V { TS x[VW]; };
+    const int64_t nv = n / VW;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += (int64_t)gridDim.x * blockDim.x) {
+        V r, a, b;
+        if (op != HSX_ARRAY_FILL) a = reinterpret_cast<const V *>(u)[i];
+        if (op == HSX_ARRAY_ADD || op == HSX_ARRAY_MULTIPLY) b = reinterpret_cast<const V *>(v)[i];
+        if (op == HSX_ARRAY_ADDTO) r = reinterpret_cast<const V *>(w)[i];
+#pragma unroll
+        for (int k = 0; k < VW; k++) {
+            switch (op) {
+            case HSX_ARRAY_FILL: r.x[k] = val; break;
+            case HSX_ARRAY_COPY: r.x[k] = a.x[k]; break;
+            case HSX_ARRAY_ADD: r.x[k] = a.x[k] + b.x[k]; break;
+            case HSX_ARRAY_MULTIPLY: r.x[k] = a.x[k] * b.x[k]; break;
+            default: r.x[k] = r.x[k] + a.x[k]; break;
+            }
+        }
+        reinterpret_cast<V *>(w)[i] = r;
+    }
+    for (int64_t i = nv * VW + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        array_elem(op, w, u, v, val, i);
+}
+
+template <typename TS>
 __global__ void __launch_bounds__(256) k_array(int op, TS *w, const TS *u, const TS *v, TS val, int64_t n)
 {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        switch (op) {
-        case HSX_ARRAY_FILL: w[i] = val; break;
-        case HSX_ARRAY_COPY: w[i] = u[i]; break;
-        case HSX_ARRAY_ADD: w[i] = u[i] + v[i]; break;
-        case HSX_ARRAY_MULTIPLY: w[i] = u[i] * v[i]; break;
-        default: w[i] = w[i] + u[i]; break;
-        }
-    }
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        array_elem(op, w, u, v, val, i);
 }
 
 // ---- free surface, one face (src/grid.cpp:3065-3230) ---------------------------
@@ -106,6 +141,73 @@ __global__ void __launch_bounds__(256) k_free_surface(TS *u, Face f)
             const int64_t i = f.hs + k - 1;
             u[base + i * f.sa] = -u[base + (2 * f.inner - i) * f.sa];
         }
+    }
+}
+
+// ---- free surface, all faces in two launches -------------------------------------
+// The six sequential phases of the reference (I1HALO1, I1HALO2, I2HALO1, ... each
+// zeroing its boundary plane then mirroring) have a closed form when every
+// active axis has at least R+2 inner points, so that a mirror source never
+// reaches the far boundary plane or a halo:
+//  * halo point p of face (a, side), mirror m:  u[p] = -0 if some face of an
+//    axis a' < a in the mask has its boundary plane through p (p and m share
+//    that coordinate; the plane was zeroed before phase a), else -u_old[m];
+//  * then every boundary plane of the mask is zero.
+// A halo point is written once and mirrors read only inner points that the
+// first launch does not write, so the first launch has no ordering inside it,
+// and the second launch (the planes) only writes inner points.
+struct FaceSet {
+    Face f[6];
+    int n;                       // faces in f
+    int64_t zidx[3][2];          // boundary plane index per axis / side
+    int zmask;                   // faces of the mask
+};
+
+template <typename TS>
+__global__ void __launch_bounds__(256) k_bc_mirror(TS *u, FaceSet fs)
+{
+    const Face &f = fs.f[blockIdx.y];
+    const int a = f.axis;
+    const int oa = a == 2 ? 1 : 2, ob = a == 0 ? 1 : 0;
+    const int64_t H = f.he - f.hs + 1;
+    const int64_t NB = f.be - f.bs + 1, NO = f.oe - f.os + 1;
+    const int64_t total = H * NB * NO;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        int64_t k, ib, io;
+        if (a == 0) {
+            k = t % H;
+            ib = (t / H) % NB;
+            io = t / (H * NB);
+        } else {
+            ib = t % NB;
+            k = (t / NB) % H;
+            io = t / (NB * H);
+        }
+        int64_t c[3];
+        c[a] = f.hs + k;
+        c[oa] = f.os + io;
+        c[ob] = f.bs + ib;
+        bool zero = false;
+        for (int ap = 0; ap < a; ap++)
+            for (int sp = 0; sp < 2; sp++)
+                if ((fs.zmask >> (2 * ap + sp)) & 1) zero |= c[ap] == fs.zidx[ap][sp];
+        const int64_t base = c[oa] * f.so + c[ob] * f.sb;
+        const int64_t i = c[a];
+        u[base + i * f.sa] = zero ? -(TS)0.0 : -u[base + (2 * f.inner - i) * f.sa];
+    }
+}
+
+template <typename TS>
+__global__ void __launch_bounds__(256) k_bc_zero(TS *u, FaceSet fs)
+{
+    const Face &f = fs.f[blockIdx.y];
+    const int64_t NB = f.be - f.bs + 1, NO = f.oe - f.os + 1;
+    const int64_t total = NB * NO;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t ib = t % NB, io = t / NB;
+        u[(f.os + io) * f.so + (f.bs + ib) * f.sb + f.inner * f.sa] = (TS)0.0;
     }
 }
 
@@ -324,13 +426,20 @@ int hsx_array_op(const hsx_grid *g, int op, void *w, const void *u, const void *
     if (op != HSX_ARRAY_FILL && (rc = check_ptr(u, "u"))) return rc;
     if ((op == HSX_ARRAY_ADD || op == HSX_ARRAY_MULTIPLY) && (rc = check_ptr(v, "v"))) return rc;
     const int64_t n = g->n1 * g->n2 * g->n3;
-    int64_t nb = (n + 255) / 256;
-    if (nb > 8192) nb = 8192;
     hipStream_t s = stream_of(stream);
+    auto a16 = [](const void *p) { return p == nullptr || ((uintptr_t)p & 15) == 0; };
+    const bool vec = a16(w) && a16(u) && a16(v);
     return dispatch_types(g, [&](auto ts, auto) -> int {
         using TS = decltype(ts);
-        hipLaunchKernelGGL(k_array<TS>, dim3((unsigned)nb), dim3(256), 0, s, op, (TS *)w, (const TS *)u,
-                           (const TS *)v, (TS)val, n);
+        const int64_t work = vec ? n / (16 / (int64_t)sizeof(TS)) + 1 : n;
+        int64_t nb = (work + 255) / 256;
+        if (nb > 8192) nb = 8192;
+        if (vec)
+            hipLaunchKernelGGL(k_array16<TS>, dim3((unsigned)nb), dim3(256), 0, s, op, (TS *)w, (const TS *)u,
+                               (const TS *)v, (TS)val, n);
+        else
+            hipLaunchKernelGGL(k_array<TS>, dim3((unsigned)nb), dim3(256), 0, s, op, (TS *)w, (const TS *)u,
+                               (const TS *)v, (TS)val, n);
         HSX_LAUNCHED("k_array");
         return HSX_OK;
     });
@@ -342,6 +451,11 @@ int hsx_free_surface(const hsx_grid *g, void *u, int face_mask, void *stream)
     if (rc) return rc;
     hipStream_t s = stream_of(stream);
     const int64_t st[3] = {1, g->n1, g->n1 * g->n2};
+    FaceSet fs;
+    fs.n = 0;
+    fs.zmask = 0;
+    for (int a = 0; a < 3; a++) fs.zidx[a][0] = fs.zidx[a][1] = -1;
+    bool closed_form = true;
     for (int axis = 0; axis < g->dim; axis++) {
         for (int side = 0; side < 2; side++) {
             if (!(face_mask & (1 << (2 * axis + side)))) continue;
@@ -365,19 +479,46 @@ int hsx_free_surface(const hsx_grid *g, void *u, int face_mask, void *stream)
             if (2 * f.inner - f.hs >= n_ax || 2 * f.inner - f.he < 0 || f.inner < 0 || f.inner >= n_ax)
                 return fail(HSX_ERR_INVALID, "free surface: mirror of axis %d face %d outside the grid", axis + 1,
                             side + 1);
-            const int64_t total = (f.he - f.hs + 2) * (f.be - f.bs + 1) * (f.oe - f.os + 1);
-            int64_t nb = (total + 255) / 256;
-            if (nb > 16384) nb = 16384;
-            rc = dispatch_types(g, [&](auto ts, auto) -> int {
-                using TS = decltype(ts);
-                hipLaunchKernelGGL(k_free_surface<TS>, dim3((unsigned)nb), dim3(256), 0, s, (TS *)u, f);
-                HSX_LAUNCHED("k_free_surface");
-                return HSX_OK;
-            });
-            if (rc) return rc;
+            fs.f[fs.n++] = f;
+            fs.zidx[axis][side] = f.inner;
+            fs.zmask |= 1 << (2 * axis + side);
         }
+        // closed form: the mirrors of each face stay strictly between the two
+        // boundary planes of the axis (both computed as the sequential phases do)
+        const int64_t z0 = g->halo1[2 * axis + 1] + 1, z1 = g->halo2[2 * axis] - 1;
+        const int64_t h1 = g->halo1[2 * axis + 1] - g->halo1[2 * axis] + 1;
+        const int64_t h2 = g->halo2[2 * axis + 1] - g->halo2[2 * axis] + 1;
+        if (z1 - z0 <= (h1 > h2 ? h1 : h2)) closed_form = false;
     }
-    return HSX_OK;
+    if (fs.n == 0) return HSX_OK;
+    return dispatch_types(g, [&](auto ts, auto) -> int {
+        using TS = decltype(ts);
+        if (closed_form) {
+            int64_t most = 0, most_plane = 0;
+            for (int k = 0; k < fs.n; k++) {
+                const Face &f = fs.f[k];
+                const int64_t plane = (f.be - f.bs + 1) * (f.oe - f.os + 1);
+                most = std::max(most, plane * (f.he - f.hs + 1));
+                most_plane = std::max(most_plane, plane);
+            }
+            const unsigned nb1 = (unsigned)std::min<int64_t>((most + 255) / 256, 4096);
+            const unsigned nb2 = (unsigned)std::min<int64_t>((most_plane + 255) / 256, 4096);
+            hipLaunchKernelGGL(k_bc_mirror<TS>, dim3(nb1, fs.n), dim3(256), 0, s, (TS *)u, fs);
+            HSX_LAUNCHED("k_bc_mirror");
+            hipLaunchKernelGGL(k_bc_zero<TS>, dim3(nb2, fs.n), dim3(256), 0, s, (TS *)u, fs);
+            HSX_LAUNCHED("k_bc_zero");
+            return HSX_OK;
+        }
+        // small grids: the reference's six phases one after the other
+        for (int k = 0; k < fs.n; k++) {
+            const Face &f = fs.f[k];
+            const int64_t total = (f.he - f.hs + 2) * (f.be - f.bs + 1) * (f.oe - f.os + 1);
+            const unsigned nb = (unsigned)std::min<int64_t>((total + 255) / 256, 16384);
+            hipLaunchKernelGGL(k_free_surface<TS>, dim3(nb), dim3(256), 0, s, (TS *)u, f);
+            HSX_LAUNCHED("k_free_surface");
+        }
+        return HSX_OK;
+    });
 }
 
 int hsx_update_pressure(const hsx_grid *g, void *prn, const void *prc, const void *coef, const void *lapla,

@@ -123,7 +123,7 @@ inline int tiled_fd_d2(const hsx_grid *g, int axis, const void *u, void *w, hipS
 {
     return axis == 0 ? tiled_fd_d2_x(g, u, w, s) : axis == 1 ? tiled_fd_d2_y(g, u, w, s) : tiled_fd_d2_z(g, u, w, s);
 }
-int set_tile(int lx, int chunk);
+int set_tile(int lx, int by, int chunk, int streams);
 
 } // namespace hsx
 

@@ -315,6 +315,9 @@ int hsx_set_kernel(int family)
 
 int hsx_get_kernel(void) { return g.family; }
 
-int hsx_set_tile(int lanes_n1, int chunk_planes) { return set_tile(lanes_n1, chunk_planes); }
+int hsx_set_tile(int lanes_n1, int rows_n2, int chunk_planes, int streams)
+{
+    return set_tile(lanes_n1, rows_n2, chunk_planes, streams);
+}
 
 } // extern "C"

@@ -23,6 +23,8 @@
 #include "hsx_internal.h"
 #include "hsx_stencil_point.h"
 
+#include <utility>
+
 namespace hsx {
 
 enum TiledMode { MODE_AC2 = 0, MODE_LAP = 1, MODE_D2X = 2, MODE_D2Y = 3, MODE_D2Z = 4 };
@@ -41,6 +43,7 @@ struct TiledArgs {
     int64_t s1, e1, s2, e2, s3, e3;
     int64_t t1s;              // n1 origin of the first tile (multiple of VW)
     int ntx, nty, nchunk, chunk;
+    int nt;                   // 1: prn / coef / output streamed with nontemporal accesses
     Coefs<TC> c;
 };
 
@@ -61,13 +64,36 @@ struct Tiled {
     static constexpr bool NEED_Z = MODE == MODE_AC2 || MODE == MODE_LAP || MODE == MODE_D2Z;
     static constexpr bool NEED_LDS = NEED_X || NEED_Y;
     using V = Pack<TS, VW>;
-    static size_t lds_bytes() { return NEED_LDS ? 2 * PLANE * sizeof(TS) : 0; }
+    static constexpr size_t LDS = NEED_LDS ? 2 * PLANE * sizeof(TS) : 0;   // two plane buffers
 };
 
 template <typename TS, int VW>
 __device__ __forceinline__ Pack<TS, VW> vload(const TS *p)
 {
     return *reinterpret_cast<const Pack<TS, VW> *>(p);
+}
+
+// the same 16 bytes as a clang vector, for the nontemporal builtins
+template <typename TS, int VW>
+using NVec = TS __attribute__((ext_vector_type(VW)));
+
+template <typename TS, int VW>
+__device__ __forceinline__ Pack<TS, VW> vload_nt(const TS *p)
+{
+    const NVec<TS, VW> t = __builtin_nontemporal_load(reinterpret_cast<const NVec<TS, VW> *>(p));
+    Pack<TS, VW> r;
+#pragma unroll
+    for (int v = 0; v < VW; v++) r.v[v] = t[v];
+    return r;
+}
+
+template <typename TS, int VW>
+__device__ __forceinline__ void vstore_nt(TS *p, const Pack<TS, VW> &x)
+{
+    NVec<TS, VW> t;
+#pragma unroll
+    for (int v = 0; v < VW; v++) t[v] = x.v[v];
+    __builtin_nontemporal_store(t, reinterpret_cast<NVec<TS, VW> *>(p));
 }
 
 template <typename TS, int VW>
@@ -79,12 +105,25 @@ __device__ __forceinline__ Pack<TS, VW> vzero()
     return z;
 }
 
+// compile-time loop: f(integral_constant<0>), ..., f(integral_constant<N-1>)
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_impl(F &f, std::integer_sequence<int, I...>)
+{
+    (f(std::integral_constant<int, I>()), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    static_for_impl(f, std::make_integer_sequence<int, N>());
+}
+
 template <typename TS, typename TC, int R, int VW, int LX, int BY, int MODE>
 __global__ void __launch_bounds__(LX *BY) k_tiled(TiledArgs<TC> a)
 {
     using K = Tiled<TS, TC, R, VW, LX, BY, MODE>;
     using V = typename K::V;
     constexpr int P = K::P, HL = K::HL, W = K::W, NHG = K::NHG, NHR = K::NHR;
+    constexpr int64_t ES = sizeof(TS);
     extern __shared__ __align__(16) unsigned char hsx_lds_raw[];
     TS *lds = reinterpret_cast<TS *>(hsx_lds_raw);
 
@@ -97,7 +136,8 @@ __global__ void __launch_bounds__(LX *BY) k_tiled(TiledArgs<TC> a)
     const int chunk = logical / (a.ntx * a.nty);
     const int tx = threadIdx.x, ty = threadIdx.y;
 
-    const int64_t i1 = a.t1s + (int64_t)(tile % a.ntx) * (LX * VW) + tx * VW;   // first n1 index of this lane
+    const int64_t t1 = a.t1s + (int64_t)(tile % a.ntx) * (LX * VW);            // first n1 index of the tile
+    const int64_t i1 = t1 + tx * VW;                                           // first n1 index of this lane
     const int64_t t2s = a.s2 + (int64_t)(tile / a.ntx) * BY;                   // first n2 row of the tile
     const int64_t i2 = t2s + ty;
     const int64_t z0 = a.s3 + (int64_t)chunk * a.chunk;
@@ -105,57 +145,71 @@ __global__ void __launch_bounds__(LX *BY) k_tiled(TiledArgs<TC> a)
 
     const bool lane_in_row = i1 + VW - 1 < a.n1;        // vector fully inside the row
     const bool row_ok = i2 < a.n2;
-    const TS *u = reinterpret_cast<const TS *>(a.u);
-    const int64_t n1 = a.n1, n12 = a.n12;
+    const bool col_ok = lane_in_row && row_ok;
+    const int64_t n1 = a.n1;
+
+    // Addressing: plane base (uniform, scalar registers) + 32-bit byte offset of
+    // the lane inside the plane (the host checks that a plane is below 4 GiB).
+    const int64_t pstride = a.n12 * ES;
+    const char *ub = reinterpret_cast<const char *>(a.u);
+    const char *pb = reinterpret_cast<const char *>(a.out);
+    const char *cb = reinterpret_cast<const char *>(a.coef);
+    char *ob = reinterpret_cast<char *>(a.out);
+    auto at = [&](const char *base, int64_t z, uint32_t off) -> const TS * {
+        return reinterpret_cast<const TS *>((base + z * pstride) + (uint64_t)off);
+    };
+    const uint32_t col = (uint32_t)((i1 + i2 * n1) * ES);
 
     // ---- queue: slot (z - z0 + R) % P holds plane z
     V q[P];
 #pragma unroll
     for (int k = 0; k < P; k++) q[k] = vzero<TS, VW>();
-    const bool col_ok = lane_in_row && row_ok;
-    const int64_t col = i1 + i2 * n1;
 #pragma unroll
-    for (int k = 0; k < 2 * R; k++) {      // planes z0-R .. z0+R-1
-        const int64_t z = z0 - R + k;
-        if (col_ok) q[k] = vload<TS, VW>(u + col + z * n12);
-    }
+    for (int k = 0; k < 2 * R; k++)        // planes z0-R .. z0+R-1
+        if (col_ok) q[k] = vload<TS, VW>(at(ub, z0 - R + k, col));
 
     // prefetch registers for the first plane
     V front = vzero<TS, VW>();
-    if (col_ok && K::NEED_Z) front = vload<TS, VW>(u + col + (z0 + R) * n12);
-    if (col_ok && !K::NEED_Z) q[R] = vload<TS, VW>(u + col + z0 * n12);
+    if (col_ok && K::NEED_Z) front = vload<TS, VW>(at(ub, z0 + R, col));
+    if (col_ok && !K::NEED_Z) q[R] = vload<TS, VW>(at(ub, z0, col));
     V hrow[NHR];          // n2 halo rows of the next plane
     V hcol = vzero<TS, VW>();   // n1 halo group of the next plane (edge lanes)
     V nprn = vzero<TS, VW>(), ncoef = vzero<TS, VW>();
 
     // which halo loads this thread owns
-    int64_t hrow_i2[NHR];
+    uint32_t hrow_off[NHR];
     int hrow_lds[NHR];
     bool hrow_ok[NHR];
 #pragma unroll
     for (int k = 0; k < NHR; k++) {
         const int h = ty + k * BY;                         // 0 .. 2R-1 for the halo rows
         const int row = h < R ? h : BY + h;                // LDS row
+        const int64_t hi2 = t2s + row - R;
         hrow_lds[k] = row;
-        hrow_i2[k] = t2s + row - R;
-        hrow_ok[k] = K::NEED_Y && h < 2 * R && hrow_i2[k] < a.n2 && lane_in_row;
+        hrow_off[k] = (uint32_t)((i1 + hi2 * n1) * ES);
+        hrow_ok[k] = K::NEED_Y && h < 2 * R && hi2 < a.n2 && lane_in_row;
         hrow[k] = vzero<TS, VW>();
     }
     const bool left = K::NEED_X && tx < NHG;
     const bool right = K::NEED_X && tx >= LX - NHG;
-    const int64_t hc_i1 = left ? a.t1s + (int64_t)(tile % a.ntx) * (LX * VW) - HL + tx * VW
-                               : a.t1s + (int64_t)(tile % a.ntx) * (LX * VW) + LX * VW + (tx - (LX - NHG)) * VW;
+    const int64_t hc_i1 = left ? t1 - HL + tx * VW : t1 + LX * VW + (tx - (LX - NHG)) * VW;
     const int hc_lds = left ? tx * VW : HL + LX * VW + (tx - (LX - NHG)) * VW;
     const bool hc_ok = (left || right) && row_ok && hc_i1 >= 0 && hc_i1 + VW - 1 < a.n1;
+    const uint32_t hc_off = (uint32_t)((hc_i1 + i2 * n1) * ES);
 
     auto load_plane_side = [&](int64_t z) {
 #pragma unroll
         for (int k = 0; k < NHR; k++)
-            if (hrow_ok[k]) hrow[k] = vload<TS, VW>(u + i1 + hrow_i2[k] * n1 + z * n12);
-        if (hc_ok) hcol = vload<TS, VW>(u + hc_i1 + i2 * n1 + z * n12);
+            if (hrow_ok[k]) hrow[k] = vload<TS, VW>(at(ub, z, hrow_off[k]));
+        if (hc_ok) hcol = vload<TS, VW>(at(ub, z, hc_off));
         if (MODE == MODE_AC2 && col_ok) {
-            nprn = vload<TS, VW>(reinterpret_cast<const TS *>(a.out) + col + z * n12);
-            ncoef = vload<TS, VW>(reinterpret_cast<const TS *>(a.coef) + col + z * n12);
+            if (a.nt) {
+                nprn = vload_nt<TS, VW>(at(pb, z, col));
+                ncoef = vload_nt<TS, VW>(at(cb, z, col));
+            } else {
+                nprn = vload<TS, VW>(at(pb, z, col));
+                ncoef = vload<TS, VW>(at(cb, z, col));
+            }
         }
     };
     load_plane_side(z0);
@@ -164,20 +218,23 @@ __global__ void __launch_bounds__(LX *BY) k_tiled(TiledArgs<TC> a)
     bool inner[VW];
 #pragma unroll
     for (int v = 0; v < VW; v++) inner[v] = i1 + v >= a.s1 && i1 + v <= a.e1 && i2 <= a.e2;
-    bool any_inner = false;
+    bool any_inner = false, all_inner = true;
 #pragma unroll
-    for (int v = 0; v < VW; v++) any_inner |= inner[v];
+    for (int v = 0; v < VW; v++) {
+        any_inner |= inner[v];
+        all_inner &= inner[v];
+    }
 
     const TC *A = a.c.A;
     int buf = 0;
     for (int64_t zb = z0; zb < z1; zb += P) {
-#pragma unroll
-        for (int s = 0; s < P; s++) {
+        // P planes per trip, each with compile-time queue slots
+        static_for<P>([&](auto sc) {
+            constexpr int s = decltype(sc)::value;
             const int64_t z = zb + s;
-            if (z >= z1) break;
-            // queue slots are compile-time after unrolling: plane z sits in (s + R) % P
-            const int cz = (s + R) % P;
-            // the front loaded during the previous plane is plane z+R: store it
+            if (z >= z1) return;                 // uniform over the workgroup
+            constexpr int cz = (s + R) % P;      // slot of plane z
+            // the front loaded during the previous plane is plane z+R
             if (K::NEED_Z) q[(s + 2 * R) % P] = front;
             const V cur = q[cz];
             const V prn_v = nprn, coef_v = ncoef;
@@ -198,15 +255,15 @@ __global__ void __launch_bounds__(LX *BY) k_tiled(TiledArgs<TC> a)
 
             // prefetch plane z+1 (queue front z+1+R, halos, prn, coef)
             if (z + 1 < z1) {
-                if (K::NEED_Z && col_ok) front = vload<TS, VW>(u + col + (z + 1 + R) * n12);
-                if (!K::NEED_Z && col_ok) q[(s + 1 + R) % P] = vload<TS, VW>(u + col + (z + 1) * n12);
+                if (K::NEED_Z && col_ok) front = vload<TS, VW>(at(ub, z + 1 + R, col));
+                if (!K::NEED_Z && col_ok) q[(s + 1 + R) % P] = vload<TS, VW>(at(ub, z + 1, col));
                 load_plane_side(z + 1);
             }
 
             if (K::NEED_LDS) __syncthreads();
 
             if (any_inner) {
-                // n1 neighbours: the row segment [HL + tx*VW - HL, HL + tx*VW + VW + HL)
+                // n1 neighbours: the row segment [tx*VW, tx*VW + VW + 2 HL) of the LDS row
                 TS xr[2 * HL + VW];
                 TS yu[R][VW], yd[R][VW];
                 if (K::NEED_X) {
@@ -275,12 +332,12 @@ __global__ void __launch_bounds__(LX *BY) k_tiled(TiledArgs<TC> a)
                     }
                     res.v[v] = o;
                 }
-                TS *outp = reinterpret_cast<TS *>(a.out) + col + z * n12;
-                bool all = true;
-#pragma unroll
-                for (int v = 0; v < VW; v++) all &= inner[v];
-                if (all) {
-                    *reinterpret_cast<V *>(outp) = res;
+                TS *outp = reinterpret_cast<TS *>((ob + z * pstride) + (uint64_t)col);
+                if (all_inner) {
+                    if (a.nt)
+                        vstore_nt<TS, VW>(outp, res);
+                    else
+                        *reinterpret_cast<V *>(outp) = res;
                 } else {
 #pragma unroll
                     for (int v = 0; v < VW; v++)
@@ -288,22 +345,42 @@ __global__ void __launch_bounds__(LX *BY) k_tiled(TiledArgs<TC> a)
                 }
             }
             buf ^= 1;
-        }
+        });
     }
 }
 
 // ---- launch configuration -------------------------------------------------------------------
 // Tile shape: LX lanes along n1 (64 = one wave per row, 16/32 for narrow grids),
-// BY = 8 rows along n2, chunk = planes per workgroup (0 = sized to give the
-// 256 CUs >= 2048 workgroups).  hsx_set_tile() overrides LX and the chunk.
-extern int g_tile_lx, g_tile_chunk;   // defined in hsx_tiled_ac2.hip
-constexpr int TILE_BY = 8;
+// BY rows along n2 (16 where the two LDS planes fit in 64 KiB, else 8), chunk =
+// planes per workgroup along n3, streams = cached or nontemporal prn / coef /
+// output.  hsx_set_tile() overrides each of them (0 = the choice made here).
+struct TileCfg {
+    int lx, by, chunk, streams;
+};
+extern TileCfg g_tile;                // defined in hsx_tiled_ac2.hip
+constexpr int TILE_LDS_MAX = 65536;
+enum { STREAMS_DEFAULT = 0, STREAMS_CACHED = 1, STREAMS_NT = 2 };
 
 inline int pick_lx(const hsx_grid *g, int vw)
 {
-    if (g_tile_lx) return g_tile_lx;
+    if (g_tile.lx) return g_tile.lx;
     const int64_t N1 = g->inner[1] - g->inner[0] + 1;
     return N1 <= 16 * vw ? 16 : N1 <= 32 * vw ? 32 : 64;
+}
+
+// planes per workgroup: short chunks keep the workgroups that share halo rows
+// in step (their re-reads hit L2) at the price of 2R warm-up planes per chunk;
+// at least 4R planes, and enough chunks for a few workgroups per CU
+inline int64_t default_chunk(int64_t N3, int R, int64_t tiles)
+{
+    int64_t len = 4 * R > 48 ? 4 * R : 48;
+    int64_t nch = (N3 + len - 1) / len;
+    if (tiles * nch < 2048) {
+        nch = (2048 + tiles - 1) / tiles;
+        len = (N3 + nch - 1) / nch;
+        if (len < 4 * R) len = N3 < 4 * R ? N3 : 4 * R;
+    }
+    return len < 1 ? 1 : len;
 }
 
 template <typename TS, typename TC, int R, int VW, int LX, int BY, int MODE>
@@ -329,21 +406,29 @@ inline int launch_tiled(const hsx_grid *g, void *out, const void *u, const void 
     a.nty = (int)((a.e2 - a.s2 + 1 + BY - 1) / BY);
     const int64_t N3 = a.e3 - a.s3 + 1;
     const int64_t tiles = (int64_t)a.ntx * a.nty;
-    // enough workgroups to fill 256 CUs a few times, chunks no shorter than 4R planes
-    int64_t nch = chunk > 0 ? (N3 + chunk - 1) / chunk : (2048 + tiles - 1) / tiles;
-    if (nch < 1) nch = 1;
-    int64_t len = (N3 + nch - 1) / nch;
-    if (chunk <= 0 && len < 4 * R) len = N3 < 4 * R ? N3 : 4 * R;
+    const int64_t len = chunk > 0 ? (chunk < N3 ? chunk : N3) : default_chunk(N3, R, tiles);
     a.chunk = (int)len;
     a.nchunk = (int)((N3 + len - 1) / len);
+    a.nt = g_tile.streams == STREAMS_NT;
     a.c = make_coefs<TC>(g);
     const int64_t nblk = tiles * a.nchunk;
     if (nblk > (int64_t)1 << 30) return fail(HSX_ERR_UNSUPPORTED, "too many tiles");
     const unsigned grid = (unsigned)(((nblk + 7) / 8) * 8);
-    static_assert(2 * K::PLANE * sizeof(TS) <= 65536 || !K::NEED_LDS, "LDS tile above 64 KiB");
-    hipLaunchKernelGGL((k_tiled<TS, TC, R, VW, LX, BY, MODE>), dim3(grid), dim3(LX, BY), K::lds_bytes(), s, a);
+    static_assert(K::LDS <= TILE_LDS_MAX, "LDS tile above 64 KiB");
+    hipLaunchKernelGGL((k_tiled<TS, TC, R, VW, LX, BY, MODE>), dim3(grid), dim3(LX, BY), K::LDS, s, a);
     HSX_LAUNCHED("k_tiled");
     return HSX_OK;
+}
+
+// BY = 16 where its LDS planes fit (and the tile is at least 32 lanes wide), else 8
+template <typename TS, typename TC, int R, int VW, int LX, int MODE>
+inline int launch_tiled_by(const hsx_grid *g, void *out, const void *u, const void *coef, hipStream_t s)
+{
+    constexpr bool tall_ok = LX >= 32 && Tiled<TS, TC, R, VW, LX, 16, MODE>::LDS <= TILE_LDS_MAX;
+    if constexpr (tall_ok) {
+        if (g_tile.by != 8) return launch_tiled<TS, TC, R, VW, LX, 16, MODE>(g, out, u, coef, g_tile.chunk, s);
+    }
+    return launch_tiled<TS, TC, R, VW, LX, 8, MODE>(g, out, u, coef, g_tile.chunk, s);
 }
 
 // alignment and size conditions of the tiled kernels
@@ -370,10 +455,9 @@ inline int run_tiled(const hsx_grid *g, void *out, const void *u, const void *co
         const int lx = pick_lx(g, VW);
         return dispatch_R(R, [&](auto rc_) -> int {
             constexpr int RR = decltype(rc_)::value;
-            constexpr int BY = TILE_BY;
-            if (lx == 16) return launch_tiled<TS, TC, RR, VW, 16, BY, MODE>(g, out, u, coef, g_tile_chunk, s);
-            if (lx == 32) return launch_tiled<TS, TC, RR, VW, 32, BY, MODE>(g, out, u, coef, g_tile_chunk, s);
-            return launch_tiled<TS, TC, RR, VW, 64, BY, MODE>(g, out, u, coef, g_tile_chunk, s);
+            if (lx == 16) return launch_tiled<TS, TC, RR, VW, 16, 8, MODE>(g, out, u, coef, g_tile.chunk, s);
+            if (lx == 32) return launch_tiled_by<TS, TC, RR, VW, 32, MODE>(g, out, u, coef, s);
+            return launch_tiled_by<TS, TC, RR, VW, 64, MODE>(g, out, u, coef, s);
         });
     });
 }

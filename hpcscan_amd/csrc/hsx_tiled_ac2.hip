@@ -3,15 +3,17 @@
 
 namespace hsx {
 
-int g_tile_lx = 0, g_tile_chunk = 0;
+TileCfg g_tile = {0, 0, 0, STREAMS_DEFAULT};
 
-int set_tile(int lx, int chunk)
+int set_tile(int lx, int by, int chunk, int streams)
 {
     if (lx != 0 && lx != 16 && lx != 32 && lx != 64)
         return fail(HSX_ERR_INVALID, "tile lanes %d not in {0,16,32,64}", lx);
+    if (by != 0 && by != 8 && by != 16) return fail(HSX_ERR_INVALID, "tile rows %d not in {0,8,16}", by);
     if (chunk < 0) return fail(HSX_ERR_INVALID, "chunk %d < 0", chunk);
-    g_tile_lx = lx;
-    g_tile_chunk = chunk;
+    if (streams < STREAMS_DEFAULT || streams > STREAMS_NT)
+        return fail(HSX_ERR_INVALID, "streams %d not in {0,1,2}", streams);
+    g_tile = TileCfg{lx, by, chunk, streams};
     return HSX_OK;
 }
 

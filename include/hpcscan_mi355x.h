@@ -95,9 +95,17 @@ int         hsx_event_record(void *event, void *stream);
 int         hsx_event_elapsed_ms(void *start, void *stop, float *ms);
 int         hsx_set_kernel(int family);
 int         hsx_get_kernel(void);
-/* tile shape of the tiled kernels: lanes along n1 (16, 32, 64; 0 = by grid
- * width) and planes per workgroup along n3 (0 = sized to fill the GPU) */
-int         hsx_set_tile(int lanes_n1, int chunk_planes);
+/* Tile shape of the tiled kernels (0 = the library's choice for each):
+ *   lanes_n1      16, 32 or 64 lanes of 16 bytes along n1 (default by grid width)
+ *   rows_n2       8 or 16 rows along n2 (16 only where two LDS planes fit in
+ *                 64 KiB and lanes_n1 >= 32; otherwise 8 is used)
+ *   chunk_planes  n3 planes streamed by one workgroup
+ *   streams       HSX_STREAMS_CACHED or HSX_STREAMS_NT: how prn, coef and the
+ *                 output are accessed (nontemporal keeps them out of the caches)
+ * Process-wide; results do not depend on it. */
+#define HSX_STREAMS_CACHED 1
+#define HSX_STREAMS_NT     2
+int         hsx_set_tile(int lanes_n1, int rows_n2, int chunk_planes, int streams);
 
 /* ---- grid filling: Grid::fill (src/grid.cpp:1650-1826) -------------------- */
 /* u[range] = val (converted to the storage type) */

@@ -92,6 +92,7 @@ void   orc_fill_sep_##SFX(const orc_grid *g, TS *u, const int64_t r[6],         
                           const double *v1, const double *v2, const double *v3,         \
                           double amp);                                                  \
 int    orc_free_surface_##SFX(const orc_grid *g, TS *u);                                \
+int    orc_free_surface_mask_##SFX(const orc_grid *g, TS *u, int face_mask);            \
 int    orc_fd_d2_##SFX(const orc_grid *g, int axis, const TS *u, TS *w,                 \
                        const TC inv2[3], orc_sched s);                                  \
 int    orc_laplacian_##SFX(const orc_grid *g, const TS *u, TS *w, const TC inv2[3],     \

@@ -100,6 +100,7 @@ def lib():
             getattr(L, f"orc_fill_sep_{sfx}").argtypes = [P(OrcGrid), vp, P(ctypes.c_int64), vp, vp,
                                                           vp, ctypes.c_double]
             getattr(L, f"orc_free_surface_{sfx}").argtypes = [P(OrcGrid), vp]
+            getattr(L, f"orc_free_surface_mask_{sfx}").argtypes = [P(OrcGrid), vp, ctypes.c_int]
             getattr(L, f"orc_fd_d2_{sfx}").argtypes = [P(OrcGrid), ctypes.c_int, vp, vp, vp, OrcSched]
             getattr(L, f"orc_laplacian_{sfx}").argtypes = [P(OrcGrid), vp, vp, vp, OrcSched]
             getattr(L, f"orc_update_pressure_{sfx}").argtypes = [P(OrcGrid), vp, vp, vp, vp]
@@ -342,8 +343,11 @@ def inv2(d, dtype, compute64=False):
     return np.array([lib().orc_inv2_f32(x) if x > 0 else np.inf for x in d], dtype=np.float32)
 
 
-def free_surface(g, u, compute64=False):
-    getattr(lib(), f"orc_free_surface_{_sfx(u.dtype, compute64)}")(ctypes.byref(g), _p(u))
+def free_surface(g, u, compute64=False, face_mask=0x3F):
+    if face_mask == 0x3F:
+        getattr(lib(), f"orc_free_surface_{_sfx(u.dtype, compute64)}")(ctypes.byref(g), _p(u))
+    else:
+        getattr(lib(), f"orc_free_surface_mask_{_sfx(u.dtype, compute64)}")(ctypes.byref(g), _p(u), face_mask)
 
 
 def fd_d2(g, axis, u, w, iv, s=None, compute64=False):

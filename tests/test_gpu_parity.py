@@ -116,6 +116,32 @@ def test_free_surface_bit_exact(oracle, capi, dim, n, order, prec):
     _same(d.download(dtype), ref)
 
 
+@pytest.mark.parametrize("n,order", [((7, 9, 6), 8), ((5, 12, 11), 8), ((9, 10, 11), 16), ((4, 4, 4), 2)])
+def test_free_surface_small_grids(oracle, capi, n, order):
+    """axes with fewer than R+2 inner points: the six phases run one after the other"""
+    p, o = _layouts(oracle, 3, n, order, np.float32)
+    u = _rand(np.random.default_rng(11), p.npoint, np.float32)
+    ref = u.copy()
+    oracle.free_surface(o, ref)
+    d = capi.DeviceArray.from_numpy(u)
+    capi.free_surface(capi.hsx_grid(p), d)
+    _same(d.download(np.float32), ref)
+
+
+@pytest.mark.parametrize("mask", [0x01, 0x02, 0x0C, 0x30, 0x15, 0x2A, 0x3E, 0x1F, 0x00])
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_free_surface_face_masks(oracle, capi, mask, prec):
+    """faces with a neighbour left out (multi-rank decomposition)"""
+    dtype, _ = PRECS[prec]
+    p, o = _layouts(oracle, 3, (14, 12, 13), 8, dtype)
+    u = _rand(np.random.default_rng(mask), p.npoint, dtype)
+    ref = u.copy()
+    oracle.free_surface(o, ref, face_mask=mask)
+    d = capi.DeviceArray.from_numpy(u)
+    capi.free_surface(capi.hsx_grid(p), d, face_mask=mask)
+    _same(d.download(dtype), ref)
+
+
 def test_free_surface_with_layers_and_padding(oracle, capi):
     for kw in (dict(nlayer=2), dict(add_pad=(3, 2, 1)), dict(offset=(1, 2, 0), mul_pad=(5, 5, 5)),
                dict(auto_pad=True)):
@@ -177,22 +203,42 @@ def test_reductions(oracle, capi, prec):
         assert mx == oracle.reduce("max", o, u, ptype=ptype)
 
 
-def test_array_ops(capi):
-    p = make_layout(3, (16, 15, 14), 4)
+@pytest.mark.parametrize("fp64", [False, True])
+@pytest.mark.parametrize("n", [(16, 15, 14), (13, 7, 5)])
+def test_array_ops(capi, fp64, n):
+    """vector path (16-byte aligned buffers, element tail when npoint is not a
+    multiple of the vector width)"""
+    p = make_layout(3, n, 4, fp64=fp64)
     g = capi.hsx_grid(p)
+    dt = np.float64 if fp64 else np.float32
     rng = np.random.default_rng(5)
-    u, v, w = (_rand(rng, p.npoint, np.float32) for _ in range(3))
+    u, v, w = (_rand(rng, p.npoint, dt) for _ in range(3))
     du, dv, dw = (capi.DeviceArray.from_numpy(x) for x in (u, v, w))
     capi.array_op(g, capi.HSX_ARRAY_ADD, dw, du, dv)
-    _same(dw.download(np.float32), u + v)
+    _same(dw.download(dt), u + v)
     capi.array_op(g, capi.HSX_ARRAY_MULTIPLY, dw, du, dv)
-    _same(dw.download(np.float32), u * v)
+    _same(dw.download(dt), u * v)
     capi.array_op(g, capi.HSX_ARRAY_ADDTO, dw, du)
-    _same(dw.download(np.float32), u * v + u)
+    _same(dw.download(dt), u * v + u)
     capi.array_op(g, capi.HSX_ARRAY_COPY, dw, du)
-    _same(dw.download(np.float32), u)
+    _same(dw.download(dt), u)
     capi.array_op(g, capi.HSX_ARRAY_FILL, dw, val=2.5)
-    _same(dw.download(np.float32), np.full_like(u, 2.5))
+    _same(dw.download(dt), np.full_like(u, 2.5))
+
+
+def test_array_ops_unaligned(capi):
+    """buffers offset by one element take the element path"""
+    p = make_layout(3, (16, 15, 14), 4)
+    g = capi.hsx_grid(p)
+    rng = np.random.default_rng(6)
+    u, v = (_rand(rng, p.npoint + 1, np.float32) for _ in range(2))
+    w = np.zeros(p.npoint + 1, np.float32)
+    du, dv, dw = (capi.DeviceArray.from_numpy(x) for x in (u, v, w))
+    off = lambda d: d.ptr.value + 4  # noqa: E731
+    capi.array_op(g, capi.HSX_ARRAY_ADD, off(dw), off(du), off(dv))
+    exp = w.copy()
+    exp[1:] = u[1:] + v[1:]
+    _same(dw.download(np.float32), exp)
 
 
 @pytest.mark.parametrize("dim,n,order", [(3, (30, 28, 26), 8), (3, (20, 20, 20), 16), (2, (60, 50), 4),
@@ -260,28 +306,35 @@ def _n1_aligned(target, order, dtype, vw):
 
 
 TILED_CASES = [
-    # (order, prec, n1 target, n2, n3, lanes, chunk)
-    (8, "f32", 72, 33, 29, 16, 5),      # several tiles in n1 and n2, chunk < 2R+1
-    (8, "f32", 72, 33, 29, 16, 9),      # chunk == 2R+1
-    (8, "f32", 130, 17, 40, 32, 13),
-    (8, "f32", 200, 24, 31, 64, 0),     # one wide tile, auto chunk
-    (2, "f32", 69, 20, 22, 16, 7),      # odd halo: n1 offset, tile origin below the inner start
-    (6, "f32", 69, 19, 23, 16, 4),
-    (16, "f32", 68, 25, 37, 16, 6),     # R = 8: two n2 halo rows per thread
-    (16, "f32", 140, 9, 20, 64, 0),
-    (4, "f64", 70, 21, 27, 16, 5),      # VW = 2
-    (12, "f64", 66, 30, 26, 32, 11),
-    (8, "f32c64", 72, 18, 21, 16, 5),   # FP32 storage, FP64 compute
+    # (order, prec, n1 target, n2, n3, lanes, chunk, rows, streams)
+    (8, "f32", 72, 33, 29, 16, 5, 0, 0),      # several tiles in n1 and n2, chunk < 2R+1
+    (8, "f32", 72, 33, 29, 16, 9, 0, 2),      # chunk == 2R+1, nontemporal streams
+    (8, "f32", 130, 17, 40, 32, 13, 8, 1),
+    (8, "f32", 130, 37, 40, 32, 13, 16, 2),   # 16 rows, partial last tile row
+    (8, "f32", 200, 24, 31, 64, 0, 8, 0),     # one wide tile, default chunk
+    (8, "f32", 200, 40, 31, 64, 7, 16, 1),
+    (2, "f32", 69, 20, 22, 16, 7, 0, 0),      # odd halo: n1 offset, tile origin below the inner start
+    (6, "f32", 69, 19, 23, 16, 4, 0, 0),
+    (6, "f32", 140, 35, 23, 32, 4, 16, 2),
+    (16, "f32", 68, 25, 37, 16, 6, 0, 0),     # R = 8: two n2 halo rows per thread at 8 rows
+    (16, "f32", 140, 9, 20, 64, 0, 8, 0),
+    (16, "f32", 140, 41, 20, 32, 0, 16, 2),   # R = 8 at 16 rows
+    (4, "f64", 70, 21, 27, 16, 5, 0, 0),      # VW = 2
+    (4, "f64", 150, 37, 27, 64, 9, 16, 2),
+    (12, "f64", 66, 30, 26, 32, 11, 0, 1),
+    (8, "f32c64", 72, 18, 21, 16, 5, 0, 0),   # FP32 storage, FP64 compute
+    (8, "f32c64", 140, 35, 21, 64, 5, 16, 2),
 ]
 
 
-@pytest.mark.parametrize("order,prec,n1,n2,n3,lanes,chunk", TILED_CASES)
-def test_tiled_shapes_bit_exact(oracle, capi, order, prec, n1, n2, n3, lanes, chunk):
+@pytest.mark.parametrize("order,prec,n1,n2,n3,lanes,chunk,rows,streams", TILED_CASES)
+def test_tiled_shapes_bit_exact(oracle, capi, order, prec, n1, n2, n3, lanes, chunk, rows, streams):
     """The LDS-tiled kernels, forced, on partial tiles and chunk boundaries."""
     dtype, c64 = PRECS[prec]
     vw = 16 // np.dtype(dtype).itemsize
     n = (_n1_aligned(n1, order, dtype, vw), n2, n3)
-    print(f"tiled order={order} prec={prec} n={n} lanes={lanes} chunk={chunk}", flush=True)
+    print(f"tiled order={order} prec={prec} n={n} lanes={lanes} chunk={chunk} rows={rows} "
+          f"streams={streams}", flush=True)
     p, o = _layouts(oracle, 3, n, order, dtype)
     assert p.n1 % vw == 0
     rng = np.random.default_rng(order * 7 + n3)
@@ -290,7 +343,7 @@ def test_tiled_shapes_bit_exact(oracle, capi, order, prec, n1, n2, n3, lanes, ch
     iv = oracle.inv2(p.d, dtype, compute64=c64)
     du, dc, dp, dw = (capi.DeviceArray.from_numpy(x) for x in (u, coef, prn0, w0))
     capi.set_kernel(capi.HSX_KERNEL_TILED)
-    capi.set_tile(lanes, chunk)
+    capi.set_tile(lanes, chunk, rows, streams)
     try:
         for axis in range(3):
             ref = w0.copy()
@@ -310,6 +363,13 @@ def test_tiled_shapes_bit_exact(oracle, capi, order, prec, n1, n2, n3, lanes, ch
     finally:
         capi.set_tile(0, 0)
         capi.set_kernel(capi.HSX_KERNEL_AUTO)
+
+
+@pytest.mark.parametrize("args", [(8, 0, 0, 0), (64, -1, 0, 0), (64, 0, 12, 0), (64, 0, 0, 3)])
+def test_set_tile_rejects_bad_shapes(capi, args):
+    lanes, chunk, rows, streams = args
+    with pytest.raises(capi.HsxError):
+        capi.set_tile(lanes, chunk, rows, streams)
 
 
 def test_tiled_rejects_misaligned_rows(capi):

@@ -261,6 +261,47 @@ def test_free_surface_invariants(oracle):
     assert np.array_equal(V[b3 + 1:e3, b2 + 1:e2, b1 + 1:e1], U[b3 + 1:e3, b2 + 1:e2, b1 + 1:e1])
 
 
+def _one_face_numpy(g, u, axis, side):
+    """one phase of the free surface, written directly on the 3D array"""
+    U = u.reshape(g.n3, g.n2, g.n1)
+    ax = g.ax
+    rng = [slice(ax[a].inner_start, ax[a].inner_end + 1) for a in range(3)]
+    b = ax[axis].inner_start if side == 0 else ax[axis].inner_end
+    h = g.fd_order // 2              # halo width R
+    def sl(i):
+        s = list(rng)
+        s[axis] = i
+        return tuple(s[::-1])        # array index order (i3, i2, i1)
+    U[sl(b)] = 0
+    for k in range(1, h + 1):
+        src = b + k if side == 0 else b - k
+        dst = b - k if side == 0 else b + k
+        U[sl(dst)] = -U[sl(src)]
+
+
+def test_free_surface_face_mask(oracle):
+    """Masked free surface = the unmasked phases in order; mask 0 is the identity."""
+    rng = np.random.default_rng(8)
+    g = oracle.layout(3, [9, 11, 10], 8)
+    u = rng.standard_normal(g.npoint).astype(np.float32)
+    v = u.copy()
+    oracle.free_surface(g, v, face_mask=0)
+    assert np.array_equal(v, u)
+    full = u.copy()
+    oracle.free_surface(g, full)
+    v = u.copy()
+    oracle.free_surface(g, v, face_mask=0x3F)
+    assert np.array_equal(v, full)
+    for mask in (0x01, 0x02, 0x08, 0x30, 0x15, 0x2A, 0x3E):
+        v = u.copy()
+        oracle.free_surface(g, v, face_mask=mask)
+        ref = u.copy()
+        for f in range(6):
+            if mask >> f & 1:
+                _one_face_numpy(g, ref, f // 2, f % 2)
+        assert np.array_equal(v, ref), hex(mask)
+
+
 def test_update_pressure_known_answer(oracle):
     """W = 2*V - W + C*L on constant grids (src/testCase_Grid.cpp:359-410)."""
     a1, a2, a3, a4 = np.float32(0.8), np.float32(0.9), np.float32(1.0), np.float32(1.1)
