@@ -71,7 +71,8 @@ def gpu_leg(args):
     capi.set_kernel({"auto": capi.HSX_KERNEL_AUTO, "ref": capi.HSX_KERNEL_REF,
                      "tiled": capi.HSX_KERNEL_TILED}[args.kernel])
     capi.set_tile(args.tile_lx, args.tile_chunk, args.tile_rows,
-                  {"default": 0, "cached": capi.HSX_STREAMS_CACHED, "nt": capi.HSX_STREAMS_NT}[args.streams])
+                  {"default": 0, "cached": capi.HSX_TILE_CACHED, "nt": capi.HSX_TILE_NT}[args.streams]
+                  | (capi.HSX_TILE_NO_RING if args.no_ring else 0))
     info = capi.device_info(0)
     n, order = args.n, args.order
     p = make_layout(3, (n, n, n), order)
@@ -138,7 +139,8 @@ def gpu_leg(args):
     alg_bytes = 16 * npt
     achieved = alg_bytes / avg_kernel_s / 1e9
     traffic = None
-    default_shape = (args.tile_lx, args.tile_rows, args.tile_chunk, args.streams) == (0, 0, 0, "default")
+    default_shape = (args.tile_lx, args.tile_rows, args.tile_chunk, args.streams, args.no_ring) == \
+        (0, 0, 0, "default", False)
     if os.path.exists(TRAFFIC_FILE) and (n, order, args.kernel) == (1000, 8, "auto") and default_shape:
         try:
             traffic = json.load(open(TRAFFIC_FILE)).get("hbm_bytes_per_launch")
@@ -216,6 +218,8 @@ def main():
                     help="n3 planes streamed per workgroup (0 = library choice)")
     ap.add_argument("--streams", default="default", choices=["default", "cached", "nt"],
                     help="cached or nontemporal prn/coef/output accesses of the fused kernel")
+    ap.add_argument("--no-ring", action="store_true",
+                    help="load each plane's halo one plane ahead instead of with the queue front")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one full-grid copy kernel first (counter calibration for profiles)")
     args = ap.parse_args()
@@ -250,7 +254,7 @@ def main():
                    "grid": [args.n] * 3, "fd_order": args.order, "propagator": "Ac2Standard",
                    "inner_points": npt, "kernel": args.kernel,
                    "tile_lx": args.tile_lx, "tile_rows": args.tile_rows, "tile_chunk": args.tile_chunk,
-                   "streams": args.streams},
+                   "streams": args.streams, "halo_ring": not args.no_ring},
         "roofline": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in gpu["roofline"].items()},
         "cpu_baseline": cpu,
         "extra": {"fused_kernel_ms": round(gpu["kernel_ms"], 4),

@@ -23,7 +23,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libhpcscan_mi355x.so")
 HSX_OK = 0
 HSX_FP32, HSX_FP64 = 0, 1
 HSX_KERNEL_AUTO, HSX_KERNEL_REF, HSX_KERNEL_TILED = 0, 1, 2
-HSX_STREAMS_CACHED, HSX_STREAMS_NT = 1, 2
+HSX_TILE_CACHED, HSX_TILE_NT, HSX_TILE_NO_RING = 1, 2, 4
 HSX_ARRAY_FILL, HSX_ARRAY_COPY, HSX_ARRAY_ADD, HSX_ARRAY_MULTIPLY, HSX_ARRAY_ADDTO = range(5)
 ALL_FACES = 0x3F
 
@@ -93,7 +93,33 @@ SIGNATURES = {
     "hsx_l1_sums": (_i, [_G, _vp, _vp, _R, _P(_d), _vp]),
     "hsx_max_err": (_i, [_G, _vp, _vp, _R, _d, _P(_d), _vp]),
     "hsx_min_max": (_i, [_G, _vp, _R, _P(_d), _vp]),
+    "hsx_ac2_step_to": (_i, [_G, _vp, _vp, _vp, _vp, _vp]),
+    "hsx_inject": (_i, [_G, _vp, _i64, _vp, _vp, _vp, _d, _i, _vp]),
+    "hsx_gather": (_i, [_G, _vp, _i64, _vp, _vp, _vp]),
+    "hsx_imaging": (_i, [_G, _vp, _vp, _vp, _vp, _vp, _R, _d, _i64, _d, _vp]),
+    "hsx_ckpt_bytes": (_sz, [_i64]),
+    "hsx_ckpt_encode": (_i, [_vp, _vp, _i64, _vp, _i, _vp]),
+    "hsx_ckpt_decode": (_i, [_vp, _i64, _i, _vp, _vp, _vp]),
+    "hsx_forward": (_i, [_G, _vp, _vp, _vp, _vp]),
+    "hsx_gradient": (_i, [_G, _vp, _vp, _vp, _vp, _vp, _P(_d), _vp]),
+    "hsx_batch_create": (_i, [_i, _G, _P(_vp), _vp, _P(_vp), _P(_vp)]),
+    "hsx_batch_run": (_i, [_vp, _i, _vp]),
+    "hsx_batch_destroy": (_i, [_vp]),
 }
+
+
+class HsxShot(ctypes.Structure):
+    _fields_ = [("nt", ctypes.c_int64), ("dt", ctypes.c_double), ("nsrc", ctypes.c_int64),
+                ("src_idx", ctypes.c_void_p), ("wavelet", ctypes.c_void_p), ("nrec", ctypes.c_int64),
+                ("rec_idx", ctypes.c_void_p)]
+
+
+class HsxAdjointOpts(ctypes.Structure):
+    _fields_ = [("ckpt_every", ctypes.c_int), ("ckpt_format", ctypes.c_int)]
+
+
+HSX_INJECT_ADD, HSX_INJECT_SET = 0, 1
+HSX_CKPT_FP32, HSX_CKPT_FP16, HSX_CKPT_BF16 = 0, 1, 2
 
 _lib = None
 
@@ -144,9 +170,10 @@ def set_kernel(family: int):
     check(lib().hsx_set_kernel(family))
 
 
-def set_tile(lanes_n1: int = 0, chunk_planes: int = 0, rows_n2: int = 0, streams: int = 0):
-    """Tile shape of the tiled kernels; 0 = the library's choice for each."""
-    check(lib().hsx_set_tile(lanes_n1, rows_n2, chunk_planes, streams))
+def set_tile(lanes_n1: int = 0, chunk_planes: int = 0, rows_n2: int = 0, flags: int = 0):
+    """Tile shape of the tiled kernels; 0 = the library's choice for each.
+    flags: HSX_TILE_CACHED or HSX_TILE_NT (streams), | HSX_TILE_NO_RING."""
+    check(lib().hsx_set_tile(lanes_n1, rows_n2, chunk_planes, flags))
 
 
 def hsx_grid(g: GridLayout, compute_fp64: Optional[bool] = None) -> HsxGrid:
@@ -286,6 +313,94 @@ def min_max(g: HsxGrid, u, r, stream=None):
 
 def sync(stream=None):
     check(lib().hsx_stream_sync(stream))
+
+
+# ---- modeling / adjoint ------------------------------------------------------------
+def ac2_step_to(g: HsxGrid, out, prn, prc, coef, stream=None):
+    check(lib().hsx_ac2_step_to(ctypes.byref(g), dptr(out), dptr(prn), dptr(prc), dptr(coef), stream))
+
+
+def inject(g: HsxGrid, u, n, idx, val, w=None, scale=1.0, mode=HSX_INJECT_ADD, stream=None):
+    check(lib().hsx_inject(ctypes.byref(g), dptr(u), int(n), dptr(idx), dptr(val), dptr(w), float(scale), mode,
+                           stream))
+
+
+def gather(g: HsxGrid, u, n, idx, out, stream=None):
+    check(lib().hsx_gather(ctypes.byref(g), dptr(u), int(n), dptr(idx), dptr(out), stream))
+
+
+def imaging(g: HsxGrid, grad, mu, u_next, u, u_prev, r, scale, src_idx=-1, src_val=0.0, stream=None):
+    check(lib().hsx_imaging(ctypes.byref(g), dptr(grad), dptr(mu), dptr(u_next), dptr(u), dptr(u_prev), rng(r),
+                            float(scale), int(src_idx), float(src_val), stream))
+
+
+def ckpt_bytes(n: int) -> int:
+    return int(lib().hsx_ckpt_bytes(int(n)))
+
+
+def ckpt_encode(u, u_prev, n, packed, fmt, stream=None):
+    check(lib().hsx_ckpt_encode(dptr(u), dptr(u_prev), int(n), dptr(packed), fmt, stream))
+
+
+def ckpt_decode(packed, n, fmt, base, out, stream=None):
+    check(lib().hsx_ckpt_decode(dptr(packed), int(n), fmt, dptr(base), dptr(out), stream))
+
+
+def make_shot(nt, dt, src_idx, wavelet, rec_idx) -> HsxShot:
+    """Shot descriptor over DEVICE arrays (DeviceArray / tensors / addresses);
+    the caller keeps them alive."""
+    s = HsxShot()
+    s.nt, s.dt = int(nt), float(dt)
+    s.nsrc = int(src_idx[1]) if isinstance(src_idx, tuple) else 0
+    s.src_idx = dptr(src_idx[0]).value if isinstance(src_idx, tuple) else None
+    s.wavelet = dptr(wavelet).value if wavelet is not None else None
+    s.nrec = int(rec_idx[1]) if isinstance(rec_idx, tuple) else 0
+    s.rec_idx = dptr(rec_idx[0]).value if isinstance(rec_idx, tuple) else None
+    return s
+
+
+class Batch:
+    """hsx_batch over lists of descriptors, coef buffers, shots and trace buffers
+    (the caller keeps the buffers alive)."""
+
+    def __init__(self, grids, coefs, shots, traces):
+        n = len(grids)
+        self._grids = (HsxGrid * n)(*grids)
+        self._coef = (ctypes.c_void_p * n)(*[dptr(c).value for c in coefs])
+        self._shots = (HsxShot * n)(*shots)
+        self._traces = (ctypes.c_void_p * n)(*[dptr(t).value for t in traces])
+        h = ctypes.c_void_p()
+        check(lib().hsx_batch_create(n, self._grids, self._coef, ctypes.cast(self._shots, ctypes.c_void_p),
+                                     self._traces, ctypes.byref(h)))
+        self.h = h
+
+    def run(self, use_graph=True, stream=None):
+        check(lib().hsx_batch_run(self.h, 1 if use_graph else 0, stream))
+
+    def close(self):
+        if self.h is not None and self.h.value:
+            lib().hsx_batch_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def forward(g: HsxGrid, coef, shot: HsxShot, traces, stream=None):
+    check(lib().hsx_forward(ctypes.byref(g), dptr(coef), ctypes.byref(shot), dptr(traces), stream))
+
+
+def gradient(g: HsxGrid, coef, shot: HsxShot, d_obs, grad, ckpt_every=0, ckpt_format=HSX_CKPT_FP32,
+             stream=None) -> float:
+    o = HsxAdjointOpts()
+    o.ckpt_every, o.ckpt_format = int(ckpt_every), int(ckpt_format)
+    mis = ctypes.c_double(0.0)
+    check(lib().hsx_gradient(ctypes.byref(g), dptr(coef), ctypes.byref(shot), dptr(d_obs), ctypes.byref(o),
+                             dptr(grad), ctypes.byref(mis), stream))
+    return mis.value
 
 
 class Event:

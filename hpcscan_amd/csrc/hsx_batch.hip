@@ -1,0 +1,294 @@
+// Batched shots: many grids (one order and precision, any sizes) advanced
+// together, one launch of each kernel per time step for all of them:
+//   free surface of every shot (2 launches), the fused Ac2 step of every shot
+//   (one k_tiled_batch launch over the tiles of all grids), then sources and
+//   receivers of every shot (one launch, one workgroup per shot).
+// The whole time loop can be captured once into a HIP graph and replayed.
+// Each shot is the forward modeling of hsx_forward (same arithmetic, same
+// results).  No counterpart in the reference, which runs one grid per process.
+#include <functional>
+#include <vector>
+
+#include "hsx_tiled.h"
+
+namespace hsx {
+
+struct SrcRec {             // one shot, one step parity
+    void *field;            // wavefield written by the step
+    const int64_t *src;
+    const void *wavelet;
+    int64_t nsrc;
+    const int64_t *rec;
+    void *traces;
+    int64_t nrec;
+};
+
+// one workgroup per shot: add the source samples of step n, then record
+template <typename TS>
+__global__ void __launch_bounds__(256) k_srcrec_batch(const SrcRec *sr, int64_t n)
+{
+    const SrcRec &x = sr[blockIdx.x];
+    TS *u = reinterpret_cast<TS *>(x.field);
+    const TS *w = reinterpret_cast<const TS *>(x.wavelet);
+    for (int64_t i = threadIdx.x; i < x.nsrc; i += blockDim.x) {
+        TS *p = u + x.src[i];
+        *p = (TS)((double)*p + 1.0 * (double)w[n * x.nsrc + i]);
+    }
+    __syncthreads();
+    TS *tr = reinterpret_cast<TS *>(x.traces);
+    for (int64_t i = threadIdx.x; i < x.nrec; i += blockDim.x) tr[n * x.nrec + i] = u[x.rec[i]];
+}
+
+template <typename TS>
+__global__ void __launch_bounds__(256) k_zero_batch(TS *const *f, const int64_t *npoint)
+{
+    TS *u = f[blockIdx.y];
+    const int64_t n = npoint[blockIdx.y];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        u[i] = (TS)0;
+}
+
+} // namespace hsx
+
+using namespace hsx;
+
+struct hsx_batch {
+    int nshots = 0;
+    int storage = HSX_FP32;
+    int64_t nt = 0;
+    std::vector<void *> owned;                 // device allocations to free
+    void *fields = nullptr;                    // [2*nshots] A_s then B_s
+    void *npoint = nullptr;                    // [2*nshots]
+    int64_t most_npoint = 0;
+    void *bc_fields[2] = {nullptr, nullptr};   // [nshots] the wavefield B applies to, per parity
+    void *sets = nullptr;                      // FaceSet per shot
+    int64_t most = 0, most_plane = 0;
+    void *sr[2] = {nullptr, nullptr};          // SrcRec per shot, per parity
+    std::function<int(int, hipStream_t)> step; // batched fused step, per parity
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t graph_stream = nullptr;
+};
+
+namespace hsx {
+
+static int dalloc(hsx_batch *b, size_t bytes, void **p)
+{
+    hipError_t e = hipMalloc(p, bytes ? bytes : 1);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc (batch)");
+    b->owned.push_back(*p);
+    return HSX_OK;
+}
+
+static int upload(hsx_batch *b, const void *host, size_t bytes, void **p)
+{
+    int rc = dalloc(b, bytes, p);
+    if (rc) return rc;
+    HSX_HIP(hipMemcpy(*p, host, bytes, hipMemcpyHostToDevice));
+    return HSX_OK;
+}
+
+// the fused-step instance shared by every shot: 64 lanes, 16 rows with the
+// halo ring where it fits, else 16 rows without it, else 8 rows
+template <typename TS, int R>
+static int batch_setup_step(hsx_batch *b, const hsx_grid *grids, const void *const *coef)
+{
+    constexpr int VW = 16 / sizeof(TS);
+    constexpr bool ring = Tiled<TS, TS, R, VW, 64, 16, MODE_AC2, true>::LDS <= TILE_LDS_MAX;
+    constexpr bool tall = Tiled<TS, TS, R, VW, 64, 16, MODE_AC2, false>::LDS <= TILE_LDS_MAX;
+    constexpr int BY = (ring || tall) ? 16 : 8;
+    using K = Tiled<TS, TS, R, VW, 64, BY, MODE_AC2, ring>;
+    auto kern = k_tiled_batch<TS, TS, R, VW, 64, BY, MODE_AC2, ring>;
+    static const int per_cu = tiled_per_cu(kern, 64 * BY, K::LDS);
+    const int ns = b->nshots;
+    const int64_t slots = std::max<int64_t>(1, (int64_t)per_cu * device_cus() / ns);
+    std::vector<TiledArgs<TS>> args[2];
+    std::vector<int> off(ns + 1, 0);
+    std::vector<void *> F(2 * ns);            // A_s then B_s
+    HSX_HIP(hipMemcpy(F.data(), b->fields, 2 * ns * sizeof(void *), hipMemcpyDeviceToHost));
+    for (int p = 0; p < 2; p++) args[p].resize(ns);
+    for (int s = 0; s < ns; s++) {
+        void *A = F[s], *B = F[ns + s];
+        // parity 0: B is the current wavefield, A the previous one (and the output)
+        int rc = fill_tiled_args<TS, TS, R, VW, 64, BY, MODE_AC2>(&grids[s], A, A, B, coef[s], slots, args[0][s]);
+        if (!rc) rc = fill_tiled_args<TS, TS, R, VW, 64, BY, MODE_AC2>(&grids[s], B, B, A, coef[s], slots, args[1][s]);
+        if (rc) return rc;
+        off[s + 1] = off[s] + args[0][s].ntx * args[0][s].nty * args[0][s].nchunk;
+    }
+    void *dargs[2], *doff;
+    int rc;
+    if ((rc = upload(b, args[0].data(), ns * sizeof(TiledArgs<TS>), &dargs[0])) ||
+        (rc = upload(b, args[1].data(), ns * sizeof(TiledArgs<TS>), &dargs[1])) ||
+        (rc = upload(b, off.data(), (ns + 1) * sizeof(int), &doff)))
+        return rc;
+    const int total = off[ns];
+    const unsigned grid = (unsigned)(((int64_t)total + 7) / 8 * 8);
+    b->step = [=](int parity, hipStream_t s) -> int {
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(64, BY), K::LDS, s, (const TiledArgs<TS> *)dargs[parity],
+                           (const int *)doff, ns);
+        HSX_LAUNCHED("k_tiled_batch");
+        return HSX_OK;
+    };
+    return HSX_OK;
+}
+
+static int batch_record(hsx_batch *b, hipStream_t s)
+{
+    const int ns = b->nshots;
+    const unsigned nbz = (unsigned)std::max<int64_t>(1, std::min<int64_t>((b->most_npoint + 255) / 256, 2048));
+    if (b->storage == HSX_FP64)
+        hipLaunchKernelGGL(k_zero_batch<double>, dim3(nbz, 2 * ns), dim3(256), 0, s, (double *const *)b->fields,
+                           (const int64_t *)b->npoint);
+    else
+        hipLaunchKernelGGL(k_zero_batch<float>, dim3(nbz, 2 * ns), dim3(256), 0, s, (float *const *)b->fields,
+                           (const int64_t *)b->npoint);
+    HSX_LAUNCHED("k_zero_batch");
+    for (int64_t n = 0; n < b->nt; n++) {
+        const int p = (int)(n & 1);
+        int rc = bc_batch_launch(b->storage, ns, (void *const *)b->bc_fields[p], (const FaceSet *)b->sets, b->most,
+                                 b->most_plane, s);
+        if (!rc) rc = b->step(p, s);
+        if (rc) return rc;
+        if (b->storage == HSX_FP64)
+            hipLaunchKernelGGL(k_srcrec_batch<double>, dim3(ns), dim3(256), 0, s, (const SrcRec *)b->sr[p], n);
+        else
+            hipLaunchKernelGGL(k_srcrec_batch<float>, dim3(ns), dim3(256), 0, s, (const SrcRec *)b->sr[p], n);
+        HSX_LAUNCHED("k_srcrec_batch");
+    }
+    return HSX_OK;
+}
+
+} // namespace hsx
+
+extern "C" {
+
+int hsx_batch_create(int nshots, const hsx_grid *grids, const void *const *coef, const hsx_shot *shots,
+                     void *const *traces, hsx_batch **out)
+{
+    if (!out) return fail(HSX_ERR_INVALID, "null batch pointer");
+    *out = nullptr;
+    if (nshots < 1 || nshots > 4096) return fail(HSX_ERR_INVALID, "nshots %d not in 1..4096", nshots);
+    int rc;
+    if ((rc = check_ptr(grids, "grids")) || (rc = check_ptr(coef, "coef")) || (rc = check_ptr(shots, "shots")) ||
+        (rc = check_ptr(traces, "traces")))
+        return rc;
+    if ((rc = ensure_init())) return rc;
+    const hsx_grid &g0 = grids[0];
+    for (int s = 0; s < nshots; s++) {
+        const hsx_grid &g = grids[s];
+        if ((rc = check_grid(&g))) return rc;
+        if (g.dim != 3 || g.fd_order != g0.fd_order || g.storage != g0.storage || g.compute != g.storage)
+            return fail(HSX_ERR_UNSUPPORTED, "batch: shot %d: every grid must be 3D with one order and precision", s);
+        const int vw = g.storage == HSX_FP64 ? 2 : 4;
+        if (g.n1 % vw) return fail(HSX_ERR_UNSUPPORTED, "batch: shot %d: n1 %% %d != 0", s, vw);
+        if (shots[s].nt != shots[0].nt) return fail(HSX_ERR_UNSUPPORTED, "batch: every shot must have the same nt");
+        if ((rc = check_ptr(coef[s], "coef[s]")) || (rc = check_ptr(traces[s], "traces[s]"))) return rc;
+        if (((uintptr_t)coef[s]) % 16) return fail(HSX_ERR_UNSUPPORTED, "batch: coef of shot %d not 16-byte aligned", s);
+    }
+    hsx_batch *b = new hsx_batch();
+    b->nshots = nshots;
+    b->storage = g0.storage;
+    b->nt = shots[0].nt;
+    const size_t es = g0.storage == HSX_FP64 ? 8 : 4;
+    std::vector<void *> F(2 * nshots);
+    std::vector<int64_t> np(2 * nshots);
+    std::vector<FaceSet> sets(nshots);
+    std::vector<SrcRec> sr[2];
+    sr[0].resize(nshots);
+    sr[1].resize(nshots);
+    std::vector<void *> bcf[2];
+    bcf[0].resize(nshots);
+    bcf[1].resize(nshots);
+    auto bail = [&](int code) {
+        hsx_batch_destroy(b);
+        return code;
+    };
+    for (int s = 0; s < nshots; s++) {
+        const hsx_grid &g = grids[s];
+        const int64_t n = g.n1 * g.n2 * g.n3;
+        void *A, *B;
+        if ((rc = dalloc(b, n * es, &A)) || (rc = dalloc(b, n * es, &B))) return bail(rc);
+        F[s] = A;
+        F[nshots + s] = B;
+        np[s] = np[nshots + s] = n;
+        b->most_npoint = std::max(b->most_npoint, n);
+        bool closed;
+        int64_t most, most_plane;
+        if ((rc = make_faceset(&g, 0x3F, &sets[s], &closed, &most, &most_plane))) return bail(rc);
+        if (!closed) return bail(fail(HSX_ERR_UNSUPPORTED, "batch: shot %d too small for the two-launch boundary", s));
+        b->most = std::max(b->most, most);
+        b->most_plane = std::max(b->most_plane, most_plane);
+        // parity 0: free surface on B, step into A; parity 1: the other way round
+        bcf[0][s] = B;
+        bcf[1][s] = A;
+        for (int p = 0; p < 2; p++) {
+            SrcRec &x = sr[p][s];
+            x.field = p == 0 ? A : B;
+            x.src = shots[s].src_idx;
+            x.wavelet = shots[s].wavelet;
+            x.nsrc = shots[s].nsrc;
+            x.rec = shots[s].rec_idx;
+            x.traces = traces[s];
+            x.nrec = shots[s].nrec;
+        }
+    }
+    if ((rc = upload(b, F.data(), F.size() * sizeof(void *), &b->fields)) ||
+        (rc = upload(b, np.data(), np.size() * sizeof(int64_t), &b->npoint)) ||
+        (rc = upload(b, sets.data(), sets.size() * sizeof(FaceSet), &b->sets)))
+        return bail(rc);
+    for (int p = 0; p < 2; p++)
+        if ((rc = upload(b, bcf[p].data(), nshots * sizeof(void *), &b->bc_fields[p])) ||
+            (rc = upload(b, sr[p].data(), nshots * sizeof(SrcRec), &b->sr[p])))
+            return bail(rc);
+    rc = dispatch_R(g0.fd_order / 2, [&](auto rc_) -> int {
+        constexpr int R = decltype(rc_)::value;
+        if (g0.storage == HSX_FP64) return batch_setup_step<double, R>(b, grids, coef);
+        return batch_setup_step<float, R>(b, grids, coef);
+    });
+    if (rc) return bail(rc);
+    *out = b;
+    return HSX_OK;
+}
+
+int hsx_batch_run(hsx_batch *b, int use_graph, void *stream)
+{
+    if (!b) return fail(HSX_ERR_INVALID, "null batch");
+    int rc = ensure_init();
+    if (rc) return rc;
+    hipStream_t s = stream_of(stream);
+    if (!use_graph) {
+        if ((rc = batch_record(b, s))) return rc;
+        HSX_HIP(hipStreamSynchronize(s));
+        return HSX_OK;
+    }
+    if (!b->exec) {
+        // capture the whole time loop once (kernels only: no memset nodes)
+        HSX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        rc = batch_record(b, s);
+        hipGraph_t gr = nullptr;
+        hipError_t e = hipStreamEndCapture(s, &gr);
+        if (rc) {
+            if (gr) (void)hipGraphDestroy(gr);
+            return rc;
+        }
+        if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+        b->graph = gr;
+        HSX_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    HSX_HIP(hipGraphLaunch(b->exec, s));
+    HSX_HIP(hipStreamSynchronize(s));
+    return HSX_OK;
+}
+
+int hsx_batch_destroy(hsx_batch *b)
+{
+    if (!b) return HSX_OK;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    (void)hipDeviceSynchronize();
+    for (void *p : b->owned) (void)hipFree(p);
+    delete b;
+    return HSX_OK;
+}
+
+} // extern "C"

@@ -109,13 +109,6 @@ __global__ void __launch_bounds__(256) k_array(int op, TS *w, const TS *u, const
 // for i in the halo range.  Sources are inner points at distance >= 1 from the
 // zeroed plane and no halo point is a source, so one face is a parallel map;
 // faces run one after the other, in the reference order.
-struct Face {
-    int axis;
-    int64_t hs, he, inner;      // halo range and zeroed plane along 'axis'
-    int64_t os, oe, bs, be;     // ranges of the outer / inner other axes
-    int64_t so, sb, sa;         // strides of outer, inner-other and face axes
-};
-
 template <typename TS>
 __global__ void __launch_bounds__(256) k_free_surface(TS *u, Face f)
 {
@@ -156,17 +149,10 @@ __global__ void __launch_bounds__(256) k_free_surface(TS *u, Face f)
 // A halo point is written once and mirrors read only inner points that the
 // first launch does not write, so the first launch has no ordering inside it,
 // and the second launch (the planes) only writes inner points.
-struct FaceSet {
-    Face f[6];
-    int n;                       // faces in f
-    int64_t zidx[3][2];          // boundary plane index per axis / side
-    int zmask;                   // faces of the mask
-};
-
 template <typename TS>
-__global__ void __launch_bounds__(256) k_bc_mirror(TS *u, FaceSet fs)
+__device__ __forceinline__ void bc_mirror_face(TS *u, const FaceSet &fs, int face)
 {
-    const Face &f = fs.f[blockIdx.y];
+    const Face &f = fs.f[face];
     const int a = f.axis;
     const int oa = a == 2 ? 1 : 2, ob = a == 0 ? 1 : 0;
     const int64_t H = f.he - f.hs + 1;
@@ -199,9 +185,9 @@ __global__ void __launch_bounds__(256) k_bc_mirror(TS *u, FaceSet fs)
 }
 
 template <typename TS>
-__global__ void __launch_bounds__(256) k_bc_zero(TS *u, FaceSet fs)
+__device__ __forceinline__ void bc_zero_face(TS *u, const FaceSet &fs, int face)
 {
-    const Face &f = fs.f[blockIdx.y];
+    const Face &f = fs.f[face];
     const int64_t NB = f.be - f.bs + 1, NO = f.oe - f.os + 1;
     const int64_t total = NB * NO;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
@@ -209,6 +195,105 @@ __global__ void __launch_bounds__(256) k_bc_zero(TS *u, FaceSet fs)
         const int64_t ib = t % NB, io = t / NB;
         u[(f.os + io) * f.so + (f.bs + ib) * f.sb + f.inner * f.sa] = (TS)0.0;
     }
+}
+
+// one grid: blockIdx.y = face
+template <typename TS>
+__global__ void __launch_bounds__(256) k_bc_mirror(TS *u, FaceSet fs)
+{
+    bc_mirror_face(u, fs, blockIdx.y);
+}
+
+template <typename TS>
+__global__ void __launch_bounds__(256) k_bc_zero(TS *u, FaceSet fs)
+{
+    bc_zero_face(u, fs, blockIdx.y);
+}
+
+// many grids: blockIdx.y = 6 * grid + face
+template <typename TS>
+__global__ void __launch_bounds__(256) k_bc_mirror_batch(TS *const *u, const FaceSet *fs)
+{
+    const int s = blockIdx.y / 6, face = blockIdx.y % 6;
+    if (face < fs[s].n) bc_mirror_face(u[s], fs[s], face);
+}
+
+template <typename TS>
+__global__ void __launch_bounds__(256) k_bc_zero_batch(TS *const *u, const FaceSet *fs)
+{
+    const int s = blockIdx.y / 6, face = blockIdx.y % 6;
+    if (face < fs[s].n) bc_zero_face(u[s], fs[s], face);
+}
+
+int make_faceset(const hsx_grid *g, int face_mask, FaceSet *fs, bool *closed_form, int64_t *most,
+                 int64_t *most_plane)
+{
+    const int64_t st[3] = {1, g->n1, g->n1 * g->n2};
+    fs->n = 0;
+    fs->zmask = 0;
+    for (int a = 0; a < 3; a++) fs->zidx[a][0] = fs->zidx[a][1] = -1;
+    *closed_form = true;
+    *most = *most_plane = 0;
+    for (int axis = 0; axis < g->dim; axis++) {
+        for (int side = 0; side < 2; side++) {
+            if (!(face_mask & (1 << (2 * axis + side)))) continue;
+            const int64_t *hr = side == 0 ? g->halo1 : g->halo2;
+            Face f;
+            f.axis = axis;
+            f.hs = hr[2 * axis];
+            f.he = hr[2 * axis + 1];
+            f.inner = side == 0 ? f.he + 1 : f.hs - 1;
+            // other axes: outer = the higher one, inner-other = the lower one
+            const int oa = axis == 2 ? 1 : 2, ob = axis == 0 ? 1 : 0;
+            f.os = g->inner[2 * oa];
+            f.oe = g->inner[2 * oa + 1];
+            f.bs = g->inner[2 * ob];
+            f.be = g->inner[2 * ob + 1];
+            f.so = st[oa];
+            f.sb = st[ob];
+            f.sa = st[axis];
+            // mirror sources must stay inside the allocation
+            const int64_t n_ax = axis == 0 ? g->n1 : axis == 1 ? g->n2 : g->n3;
+            if (2 * f.inner - f.hs >= n_ax || 2 * f.inner - f.he < 0 || f.inner < 0 || f.inner >= n_ax)
+                return fail(HSX_ERR_INVALID, "free surface: mirror of axis %d face %d outside the grid", axis + 1,
+                            side + 1);
+            fs->f[fs->n++] = f;
+            fs->zidx[axis][side] = f.inner;
+            fs->zmask |= 1 << (2 * axis + side);
+            const int64_t plane = (f.be - f.bs + 1) * (f.oe - f.os + 1);
+            *most = std::max(*most, plane * (f.he - f.hs + 1));
+            *most_plane = std::max(*most_plane, plane);
+        }
+        // closed form: the mirrors of each face stay strictly between the two
+        // boundary planes of the axis (both computed as the sequential phases do)
+        const int64_t z0 = g->halo1[2 * axis + 1] + 1, z1 = g->halo2[2 * axis] - 1;
+        const int64_t h1 = g->halo1[2 * axis + 1] - g->halo1[2 * axis] + 1;
+        const int64_t h2 = g->halo2[2 * axis + 1] - g->halo2[2 * axis] + 1;
+        if (z1 - z0 <= (h1 > h2 ? h1 : h2)) *closed_form = false;
+    }
+    return HSX_OK;
+}
+
+int bc_batch_launch(int storage, int nshots, void *const *d_fields, const FaceSet *d_sets, int64_t most,
+                    int64_t most_plane, hipStream_t s)
+{
+    const unsigned nb1 = (unsigned)std::max<int64_t>(1, std::min<int64_t>((most + 255) / 256, 4096));
+    const unsigned nb2 = (unsigned)std::max<int64_t>(1, std::min<int64_t>((most_plane + 255) / 256, 4096));
+    if (storage == HSX_FP64) {
+        hipLaunchKernelGGL(k_bc_mirror_batch<double>, dim3(nb1, 6 * nshots), dim3(256), 0, s,
+                           (double *const *)d_fields, d_sets);
+        HSX_LAUNCHED("k_bc_mirror_batch");
+        hipLaunchKernelGGL(k_bc_zero_batch<double>, dim3(nb2, 6 * nshots), dim3(256), 0, s,
+                           (double *const *)d_fields, d_sets);
+    } else {
+        hipLaunchKernelGGL(k_bc_mirror_batch<float>, dim3(nb1, 6 * nshots), dim3(256), 0, s,
+                           (float *const *)d_fields, d_sets);
+        HSX_LAUNCHED("k_bc_mirror_batch");
+        hipLaunchKernelGGL(k_bc_zero_batch<float>, dim3(nb2, 6 * nshots), dim3(256), 0, s,
+                           (float *const *)d_fields, d_sets);
+    }
+    HSX_LAUNCHED("k_bc_zero_batch");
+    return HSX_OK;
 }
 
 // ---- leapfrog update of the split propagator (src/grid.cpp:4579-4610) -----------
@@ -450,57 +535,14 @@ int hsx_free_surface(const hsx_grid *g, void *u, int face_mask, void *stream)
     int rc = common_checks(g, {{u, "u"}});
     if (rc) return rc;
     hipStream_t s = stream_of(stream);
-    const int64_t st[3] = {1, g->n1, g->n1 * g->n2};
     FaceSet fs;
-    fs.n = 0;
-    fs.zmask = 0;
-    for (int a = 0; a < 3; a++) fs.zidx[a][0] = fs.zidx[a][1] = -1;
-    bool closed_form = true;
-    for (int axis = 0; axis < g->dim; axis++) {
-        for (int side = 0; side < 2; side++) {
-            if (!(face_mask & (1 << (2 * axis + side)))) continue;
-            const int64_t *hr = side == 0 ? g->halo1 : g->halo2;
-            Face f;
-            f.axis = axis;
-            f.hs = hr[2 * axis];
-            f.he = hr[2 * axis + 1];
-            f.inner = side == 0 ? f.he + 1 : f.hs - 1;
-            // other axes: outer = the higher one, inner-other = the lower one
-            const int oa = axis == 2 ? 1 : 2, ob = axis == 0 ? 1 : 0;
-            f.os = g->inner[2 * oa];
-            f.oe = g->inner[2 * oa + 1];
-            f.bs = g->inner[2 * ob];
-            f.be = g->inner[2 * ob + 1];
-            f.so = st[oa];
-            f.sb = st[ob];
-            f.sa = st[axis];
-            // mirror sources must stay inside the allocation
-            const int64_t n_ax = axis == 0 ? g->n1 : axis == 1 ? g->n2 : g->n3;
-            if (2 * f.inner - f.hs >= n_ax || 2 * f.inner - f.he < 0 || f.inner < 0 || f.inner >= n_ax)
-                return fail(HSX_ERR_INVALID, "free surface: mirror of axis %d face %d outside the grid", axis + 1,
-                            side + 1);
-            fs.f[fs.n++] = f;
-            fs.zidx[axis][side] = f.inner;
-            fs.zmask |= 1 << (2 * axis + side);
-        }
-        // closed form: the mirrors of each face stay strictly between the two
-        // boundary planes of the axis (both computed as the sequential phases do)
-        const int64_t z0 = g->halo1[2 * axis + 1] + 1, z1 = g->halo2[2 * axis] - 1;
-        const int64_t h1 = g->halo1[2 * axis + 1] - g->halo1[2 * axis] + 1;
-        const int64_t h2 = g->halo2[2 * axis + 1] - g->halo2[2 * axis] + 1;
-        if (z1 - z0 <= (h1 > h2 ? h1 : h2)) closed_form = false;
-    }
+    bool closed_form;
+    int64_t most, most_plane;
+    if ((rc = make_faceset(g, face_mask, &fs, &closed_form, &most, &most_plane))) return rc;
     if (fs.n == 0) return HSX_OK;
     return dispatch_types(g, [&](auto ts, auto) -> int {
         using TS = decltype(ts);
         if (closed_form) {
-            int64_t most = 0, most_plane = 0;
-            for (int k = 0; k < fs.n; k++) {
-                const Face &f = fs.f[k];
-                const int64_t plane = (f.be - f.bs + 1) * (f.oe - f.os + 1);
-                most = std::max(most, plane * (f.he - f.hs + 1));
-                most_plane = std::max(most_plane, plane);
-            }
             const unsigned nb1 = (unsigned)std::min<int64_t>((most + 255) / 256, 4096);
             const unsigned nb2 = (unsigned)std::min<int64_t>((most_plane + 255) / 256, 4096);
             hipLaunchKernelGGL(k_bc_mirror<TS>, dim3(nb1, fs.n), dim3(256), 0, s, (TS *)u, fs);

@@ -51,6 +51,27 @@ struct Box {                    // inclusive index box
 };
 inline Box box_of(const int64_t r[6]) { return Box{r[0], r[1], r[2], r[3], r[4], r[5]}; }
 
+// ---- free-surface faces (src/grid.cpp:3065-3230) ----------------------------------
+struct Face {
+    int axis;
+    int64_t hs, he, inner;      // halo range and zeroed plane along 'axis'
+    int64_t os, oe, bs, be;     // ranges of the outer / inner other axes
+    int64_t so, sb, sa;         // strides of outer, inner-other and face axes
+};
+struct FaceSet {
+    Face f[6];
+    int n;                       // faces in f
+    int64_t zidx[3][2];          // boundary plane index per axis / side
+    int zmask;                   // faces of the mask
+};
+// faces of face_mask; closed_form: the two-launch form applies; most / most_plane:
+// largest halo slab and boundary plane (points), for the launch sizes
+int make_faceset(const hsx_grid *g, int face_mask, FaceSet *fs, bool *closed_form, int64_t *most,
+                 int64_t *most_plane);
+// free surface of many grids (closed form only): d_fields / d_sets device arrays
+int bc_batch_launch(int storage, int nshots, void *const *d_fields, const FaceSet *d_sets, int64_t most,
+                    int64_t most_plane, hipStream_t s);
+
 // ---- coefficients in the compute type (src/fdm.h:276-507, src/grid.cpp:3251-3257) --
 template <typename TC>
 struct Coefs {
@@ -114,7 +135,7 @@ inline int dispatch_dim(int dim, F &&f)
 
 // tiled kernels (hsx_stencil_tiled.hip); return HSX_ERR_UNSUPPORTED when the
 // grid does not fit their assumptions so the caller uses the reference kernels
-int tiled_ac2_step(const hsx_grid *g, void *prn, const void *prc, const void *coef, hipStream_t s);
+int tiled_ac2_step(const hsx_grid *g, void *out, const void *prn, const void *prc, const void *coef, hipStream_t s);
 int tiled_laplacian(const hsx_grid *g, const void *u, void *w, hipStream_t s);
 int tiled_fd_d2_x(const hsx_grid *g, const void *u, void *w, hipStream_t s);
 int tiled_fd_d2_y(const hsx_grid *g, const void *u, void *w, hipStream_t s);

@@ -32,6 +32,7 @@ struct State {
     int device = 0;
     hipStream_t stream = nullptr;
     int family = HSX_KERNEL_AUTO;
+    int cus = 0;                  // compute units of the device
     void *scratch[SCRATCH_SLOTS] = {nullptr};
     size_t scratch_bytes[SCRATCH_SLOTS] = {0};
 };
@@ -56,10 +57,13 @@ static int init_locked(int device)
     }
     // default priority on purpose: streams of mixed priorities are not used here
     HSX_HIP(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+    HSX_HIP(hipDeviceGetAttribute(&g.cus, hipDeviceAttributeMultiprocessorCount, device));
     g.device = device;
     g.ready = true;
     return HSX_OK;
 }
+
+int device_cus(void) { return g.cus > 0 ? g.cus : 1; }
 
 int ensure_init(void)
 {
@@ -315,9 +319,9 @@ int hsx_set_kernel(int family)
 
 int hsx_get_kernel(void) { return g.family; }
 
-int hsx_set_tile(int lanes_n1, int rows_n2, int chunk_planes, int streams)
+int hsx_set_tile(int lanes_n1, int rows_n2, int chunk_planes, int flags)
 {
-    return set_tile(lanes_n1, rows_n2, chunk_planes, streams);
+    return set_tile(lanes_n1, rows_n2, chunk_planes, flags);
 }
 
 } // extern "C"

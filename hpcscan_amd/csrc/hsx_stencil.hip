@@ -40,8 +40,9 @@ __global__ void __launch_bounds__(RX *RY) k_laplacian_ref(const TS *__restrict__
     w[ii] = (TS)laplacian_point<TS, TC, R, DIM>(u, ii, n1, n12, c);
 }
 
+// out = 2 prc - prn + coef * Lap(prc); out may be prn (the in-place step)
 template <typename TS, typename TC, int R, int DIM>
-__global__ void __launch_bounds__(RX *RY) k_ac2_ref(TS *__restrict__ prn, const TS *__restrict__ prc,
+__global__ void __launch_bounds__(RX *RY) k_ac2_ref(TS *out, const TS *prn, const TS *__restrict__ prc,
                                                     const TS *__restrict__ coef, Box b, int64_t n1, int64_t n12,
                                                     Coefs<TC> c)
 {
@@ -51,7 +52,7 @@ __global__ void __launch_bounds__(RX *RY) k_ac2_ref(TS *__restrict__ prn, const 
     if (i1 > b.e1 || i2 > b.e2) return;
     const int64_t ii = i1 + i2 * n1 + i3 * n12;
     const TC lap = laplacian_point<TS, TC, R, DIM>(prc, ii, n1, n12, c);
-    prn[ii] = leapfrog<TS, TC>(prc[ii], prn[ii], coef[ii], lap);
+    out[ii] = leapfrog<TS, TC>(prc[ii], prn[ii], coef[ii], lap);
 }
 
 static int ref_grid(const Box &b, dim3 &grid)
@@ -135,12 +136,17 @@ int hsx_laplacian(const hsx_grid *g, const void *u, void *w, void *stream)
 
 int hsx_ac2_step(const hsx_grid *g, void *prn, const void *prc, const void *coef, void *stream)
 {
-    int rc = stencil_checks(g, {{prn, "prn"}, {prc, "prc"}, {coef, "coef"}});
+    return hsx_ac2_step_to(g, prn, prn, prc, coef, stream);
+}
+
+int hsx_ac2_step_to(const hsx_grid *g, void *out, const void *prn, const void *prc, const void *coef, void *stream)
+{
+    int rc = stencil_checks(g, {{out, "out"}, {prn, "prn"}, {prc, "prc"}, {coef, "coef"}});
     if (rc) return rc;
-    if (prn == prc) return fail(HSX_ERR_INVALID, "prn and prc must be different grids");
+    if (prn == prc || out == prc) return fail(HSX_ERR_INVALID, "prn / out and prc must be different grids");
     hipStream_t s = stream_of(stream);
     if (kernel_family() != HSX_KERNEL_REF) {
-        rc = tiled_ac2_step(g, prn, prc, coef, s);
+        rc = tiled_ac2_step(g, out, prn, prc, coef, s);
         if (rc != HSX_ERR_UNSUPPORTED || kernel_family() == HSX_KERNEL_TILED) return rc;
     }
     const Box b = box_of(g->inner);
@@ -154,8 +160,8 @@ int hsx_ac2_step(const hsx_grid *g, void *prn, const void *prc, const void *coef
             constexpr int R = decltype(rc_)::value;
             return dispatch_dim(g->dim, [&](auto dc) -> int {
                 constexpr int DIM = decltype(dc)::value;
-                hipLaunchKernelGGL((k_ac2_ref<TS, TC, R, DIM>), grid, dim3(RX, RY), 0, s, (TS *)prn,
-                                   (const TS *)prc, (const TS *)coef, b, g->n1, g->n1 * g->n2, c);
+                hipLaunchKernelGGL((k_ac2_ref<TS, TC, R, DIM>), grid, dim3(RX, RY), 0, s, (TS *)out,
+                                   (const TS *)prn, (const TS *)prc, (const TS *)coef, b, g->n1, g->n1 * g->n2, c);
                 HSX_LAUNCHED("k_ac2_ref");
                 return HSX_OK;
             });

@@ -23,6 +23,7 @@
 #include "hsx_internal.h"
 #include "hsx_stencil_point.h"
 
+#include <algorithm>
 #include <utility>
 
 namespace hsx {
@@ -36,7 +37,8 @@ struct alignas(sizeof(T) * VW) Pack {
 
 template <typename TC>
 struct TiledArgs {
-    void *out;                // prn (MODE_AC2) or w
+    void *out;                // new prn (MODE_AC2) or w
+    const void *prn_in;       // MODE_AC2: old prn (may be out)
     const void *u;            // prc / u
     const void *coef;         // MODE_AC2 only
     int64_t n1, n2, n12;      // row length, rows per plane, plane stride
@@ -50,7 +52,7 @@ struct TiledArgs {
 constexpr int ceil_div(int a, int b) { return (a + b - 1) / b; }
 constexpr int round_up(int a, int b) { return ceil_div(a, b) * b; }
 
-template <typename TS, typename TC, int R, int VW, int LX, int BY, int MODE>
+template <typename TS, typename TC, int R, int VW, int LX, int BY, int MODE, bool RING_ = false>
 struct Tiled {
     static constexpr int P = 2 * R + 1;               // queue length
     static constexpr int HL = round_up(R, VW);        // halo slots left / right (multiple of VW)
@@ -64,7 +66,16 @@ struct Tiled {
     static constexpr bool NEED_Z = MODE == MODE_AC2 || MODE == MODE_LAP || MODE == MODE_D2Z;
     static constexpr bool NEED_LDS = NEED_X || NEED_Y;
     using V = Pack<TS, VW>;
-    static constexpr size_t LDS = NEED_LDS ? 2 * PLANE * sizeof(TS) : 0;   // two plane buffers
+    // Halo ring: the n1 / n2 halo of plane z+R is loaded together with the
+    // queue front (plane z+R), i.e. while the neighbouring tiles load the same
+    // rows as their own queue front, so those loads meet in L2; it waits in an
+    // LDS ring of R+1 planes, private to the thread that loaded it.
+    static constexpr bool RING = RING_ && NEED_X && NEED_Y && NEED_Z;
+    static constexpr int NR = R + 1;
+    static constexpr size_t PLANES_BYTES = NEED_LDS ? 2 * PLANE * sizeof(TS) : 0;   // two plane buffers
+    static constexpr size_t RING_H = RING ? (size_t)NR * 2 * R * LX * sizeof(V) : 0;
+    static constexpr size_t RING_C = RING ? (size_t)NR * BY * 2 * NHG * sizeof(V) : 0;
+    static constexpr size_t LDS = PLANES_BYTES + RING_H + RING_C;
 };
 
 template <typename TS, int VW>
@@ -117,21 +128,29 @@ __device__ __forceinline__ void static_for(F &&f)
     static_for_impl(f, std::make_integer_sequence<int, N>());
 }
 
-template <typename TS, typename TC, int R, int VW, int LX, int BY, int MODE>
-__global__ void __launch_bounds__(LX *BY) k_tiled(TiledArgs<TC> a)
+// XCD-aware block order: the hardware deals workgroups to the 8 XCDs round
+// robin; logical = (blockIdx % 8) * per_xcd + blockIdx / 8 gives each XCD (one
+// L2) a contiguous run of logical blocks, i.e. neighbouring tiles
+__device__ __forceinline__ int xcd_logical(int total)
 {
-    using K = Tiled<TS, TC, R, VW, LX, BY, MODE>;
+    const int per_xcd = (total + 7) / 8;
+    return (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+}
+
+// one workgroup: tile / chunk number 'logical' of the grid described by a
+template <typename TS, typename TC, int R, int VW, int LX, int BY, int MODE, bool RING_>
+__device__ __forceinline__ void tiled_body(const TiledArgs<TC> &a, int logical)
+{
+    using K = Tiled<TS, TC, R, VW, LX, BY, MODE, RING_>;
     using V = typename K::V;
-    constexpr int P = K::P, HL = K::HL, W = K::W, NHG = K::NHG, NHR = K::NHR;
+    constexpr int P = K::P, HL = K::HL, W = K::W, NHG = K::NHG, NHR = K::NHR, NR = K::NR;
+    constexpr bool RING = K::RING;
     constexpr int64_t ES = sizeof(TS);
     extern __shared__ __align__(16) unsigned char hsx_lds_raw[];
     TS *lds = reinterpret_cast<TS *>(hsx_lds_raw);
+    V *ring_h = reinterpret_cast<V *>(hsx_lds_raw + K::PLANES_BYTES);
+    V *ring_c = reinterpret_cast<V *>(hsx_lds_raw + K::PLANES_BYTES + K::RING_H);
 
-    // ---- XCD-aware block -> (chunk, tile) map: consecutive logical blocks share an XCD
-    const int nblk = a.ntx * a.nty * a.nchunk;
-    const int per_xcd = (nblk + 7) / 8;
-    const int logical = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
-    if (logical >= nblk) return;                       // whole block leaves, before any barrier
     const int tile = logical % (a.ntx * a.nty);
     const int chunk = logical / (a.ntx * a.nty);
     const int tx = threadIdx.x, ty = threadIdx.y;
@@ -152,7 +171,7 @@ __global__ void __launch_bounds__(LX *BY) k_tiled(TiledArgs<TC> a)
     // the lane inside the plane (the host checks that a plane is below 4 GiB).
     const int64_t pstride = a.n12 * ES;
     const char *ub = reinterpret_cast<const char *>(a.u);
-    const char *pb = reinterpret_cast<const char *>(a.out);
+    const char *pb = reinterpret_cast<const char *>(a.prn_in);
     const char *cb = reinterpret_cast<const char *>(a.coef);
     char *ob = reinterpret_cast<char *>(a.out);
     auto at = [&](const char *base, int64_t z, uint32_t off) -> const TS * {
@@ -197,11 +216,24 @@ __global__ void __launch_bounds__(LX *BY) k_tiled(TiledArgs<TC> a)
     const bool hc_ok = (left || right) && row_ok && hc_i1 >= 0 && hc_i1 + VW - 1 < a.n1;
     const uint32_t hc_off = (uint32_t)((hc_i1 + i2 * n1) * ES);
 
-    auto load_plane_side = [&](int64_t z) {
+    // ring slots owned by this thread
+    const int rc_idx = left ? tx : NHG + (tx - (LX - NHG));
+    auto ring_h_at = [&](int slot, int k) -> V * { return ring_h + ((slot * 2 * R + ty + k * BY) * LX + tx); };
+    auto ring_c_at = [&](int slot) -> V * { return ring_c + ((slot * BY + ty) * 2 * NHG + rc_idx); };
+
+    auto load_halo = [&](int64_t z) {
 #pragma unroll
         for (int k = 0; k < NHR; k++)
             if (hrow_ok[k]) hrow[k] = vload<TS, VW>(at(ub, z, hrow_off[k]));
         if (hc_ok) hcol = vload<TS, VW>(at(ub, z, hc_off));
+    };
+    auto store_halo = [&](int slot) {
+#pragma unroll
+        for (int k = 0; k < NHR; k++)
+            if (hrow_ok[k]) *ring_h_at(slot, k) = hrow[k];
+        if (hc_ok) *ring_c_at(slot) = hcol;
+    };
+    auto load_streams = [&](int64_t z) {
         if (MODE == MODE_AC2 && col_ok) {
             if (a.nt) {
                 nprn = vload_nt<TS, VW>(at(pb, z, col));
@@ -212,7 +244,17 @@ __global__ void __launch_bounds__(LX *BY) k_tiled(TiledArgs<TC> a)
             }
         }
     };
-    load_plane_side(z0);
+    if (RING) {
+        // halos of planes z0 .. z0+R-1 into the ring, plane z0+R in registers
+        for (int j = 0; j < R; j++) {
+            load_halo(z0 + j);
+            store_halo((int)((z0 + j) % NR));
+        }
+        load_halo(z0 + R);
+    } else {
+        load_halo(z0);
+    }
+    load_streams(z0);
 
     // compute mask: which of the VW points of this lane are inner points
     bool inner[VW];
@@ -227,6 +269,7 @@ __global__ void __launch_bounds__(LX *BY) k_tiled(TiledArgs<TC> a)
 
     const TC *A = a.c.A;
     int buf = 0;
+    int rslot = (int)(z0 % NR);              // ring slot of plane z
     for (int64_t zb = z0; zb < z1; zb += P) {
         // P planes per trip, each with compile-time queue slots
         static_for<P>([&](auto sc) {
@@ -239,9 +282,21 @@ __global__ void __launch_bounds__(LX *BY) k_tiled(TiledArgs<TC> a)
             const V cur = q[cz];
             const V prn_v = nprn, coef_v = ncoef;
             V hrow_cur[NHR];
+            V hcol_cur;
+            if (RING) {
+                // halo of plane z from the ring, then the halo of plane z+R
+                // (registers) into the slot of plane z-1, free since then
 #pragma unroll
-            for (int k = 0; k < NHR; k++) hrow_cur[k] = hrow[k];
-            const V hcol_cur = hcol;
+                for (int k = 0; k < NHR; k++)
+                    if (hrow_ok[k]) hrow_cur[k] = *ring_h_at(rslot, k);
+                if (hc_ok) hcol_cur = *ring_c_at(rslot);
+                store_halo(rslot == 0 ? NR - 1 : rslot - 1);
+                rslot = rslot == NR - 1 ? 0 : rslot + 1;
+            } else {
+#pragma unroll
+                for (int k = 0; k < NHR; k++) hrow_cur[k] = hrow[k];
+                hcol_cur = hcol;
+            }
 
             // stage plane z in LDS
             TS *pl = lds + buf * K::PLANE;
@@ -253,11 +308,13 @@ __global__ void __launch_bounds__(LX *BY) k_tiled(TiledArgs<TC> a)
                 if (hc_ok) *reinterpret_cast<V *>(pl + (ty + R) * W + hc_lds) = hcol_cur;
             }
 
-            // prefetch plane z+1 (queue front z+1+R, halos, prn, coef)
+            // prefetch plane z+1: queue front z+1+R, halos (of z+1+R with the
+            // ring, else of z+1), prn and coef
             if (z + 1 < z1) {
                 if (K::NEED_Z && col_ok) front = vload<TS, VW>(at(ub, z + 1 + R, col));
                 if (!K::NEED_Z && col_ok) q[(s + 1 + R) % P] = vload<TS, VW>(at(ub, z + 1, col));
-                load_plane_side(z + 1);
+                load_halo(RING ? z + 1 + R : z + 1);
+                load_streams(z + 1);
             }
 
             if (K::NEED_LDS) __syncthreads();
@@ -349,17 +406,46 @@ __global__ void __launch_bounds__(LX *BY) k_tiled(TiledArgs<TC> a)
     }
 }
 
+template <typename TS, typename TC, int R, int VW, int LX, int BY, int MODE, bool RING_>
+__global__ void __launch_bounds__(LX *BY) k_tiled(TiledArgs<TC> a)
+{
+    const int nblk = a.ntx * a.nty * a.nchunk;
+    const int logical = xcd_logical(nblk);
+    if (logical >= nblk) return;                       // whole block leaves, before any barrier
+    tiled_body<TS, TC, R, VW, LX, BY, MODE, RING_>(a, logical);
+}
+
+// Many grids in one launch (the batched shots): off[s] = first logical block
+// of grid s (off[nshots] = total); the workgroup finds its grid, then runs the
+// same tile code on that grid's arguments.
+template <typename TS, typename TC, int R, int VW, int LX, int BY, int MODE, bool RING_>
+__global__ void __launch_bounds__(LX *BY) k_tiled_batch(const TiledArgs<TC> *__restrict__ grids,
+                                                      const int *__restrict__ off, int nshots)
+{
+    const int total = off[nshots];
+    const int logical = xcd_logical(total);
+    if (logical >= total) return;
+    int s = 0;
+    while (s + 1 < nshots && off[s + 1] <= logical) s++;
+    const TiledArgs<TC> a = grids[s];
+    tiled_body<TS, TC, R, VW, LX, BY, MODE, RING_>(a, logical - off[s]);
+}
+
 // ---- launch configuration -------------------------------------------------------------------
 // Tile shape: LX lanes along n1 (64 = one wave per row, 16/32 for narrow grids),
-// BY rows along n2 (16 where the two LDS planes fit in 64 KiB, else 8), chunk =
-// planes per workgroup along n3, streams = cached or nontemporal prn / coef /
-// output.  hsx_set_tile() overrides each of them (0 = the choice made here).
+// BY rows along n2, chunk = planes per workgroup along n3, streams = cached or
+// nontemporal prn / coef / output, halo ring on or off.  hsx_set_tile()
+// overrides each of them (0 = the choice made here):
+//  * BY = 16 with the halo ring where its LDS fits (LX >= 32), else BY = 8;
+//  * 16-row tiles: one chunk per tile when the tiles fill the device once
+//    (every workgroup starts together, so tiles that share halo rows stream
+//    through L2 in step), nontemporal streams;
+//  * 8-row tiles: short chunks, cached streams.
 struct TileCfg {
-    int lx, by, chunk, streams;
+    int lx, by, chunk, flags;
 };
 extern TileCfg g_tile;                // defined in hsx_tiled_ac2.hip
-constexpr int TILE_LDS_MAX = 65536;
-enum { STREAMS_DEFAULT = 0, STREAMS_CACHED = 1, STREAMS_NT = 2 };
+constexpr int TILE_LDS_MAX = 163840;  // gfx950: 160 KiB of LDS per workgroup
 
 inline int pick_lx(const hsx_grid *g, int vw)
 {
@@ -368,32 +454,59 @@ inline int pick_lx(const hsx_grid *g, int vw)
     return N1 <= 16 * vw ? 16 : N1 <= 32 * vw ? 32 : 64;
 }
 
-// planes per workgroup: short chunks keep the workgroups that share halo rows
-// in step (their re-reads hit L2) at the price of 2R warm-up planes per chunk;
-// at least 4R planes, and enough chunks for a few workgroups per CU
-inline int64_t default_chunk(int64_t N3, int R, int64_t tiles)
+// 8-row tiles: chunks of ~48 planes (at least 4R), at least 2048 workgroups
+inline int64_t chunk_short(int64_t N3, int R, int64_t tiles)
 {
     int64_t len = 4 * R > 48 ? 4 * R : 48;
     int64_t nch = (N3 + len - 1) / len;
     if (tiles * nch < 2048) {
         nch = (2048 + tiles - 1) / tiles;
         len = (N3 + nch - 1) / nch;
-        if (len < 4 * R) len = N3 < 4 * R ? N3 : 4 * R;
     }
-    return len < 1 ? 1 : len;
+    if (len < 4 * R) len = 4 * R;
+    return len < N3 ? len : N3;
 }
 
-template <typename TS, typename TC, int R, int VW, int LX, int BY, int MODE>
-inline int launch_tiled(const hsx_grid *g, void *out, const void *u, const void *coef, int chunk, hipStream_t s)
+// 16-row tiles: as many chunks per tile as make tiles x chunks one full round
+// of resident workgroups (at least 4R planes per chunk)
+inline int64_t chunk_one_round(int64_t N3, int R, int64_t tiles, int64_t slots)
 {
-    using K = Tiled<TS, TC, R, VW, LX, BY, MODE>;
-    TiledArgs<TC> a;
+    int64_t nch = (slots + tiles / 2) / tiles;
+    if (nch < 1) nch = 1;
+    int64_t len = (N3 + nch - 1) / nch;
+    if (len < 4 * R) len = 4 * R;
+    return len < N3 ? len : N3;
+}
+
+int device_cus(void);   // compute units of the current device (hsx_runtime.hip)
+
+// resident workgroups per CU of a kernel instance (once per instantiation;
+// also lifts the 64 KiB default cap on dynamic LDS where the tile needs more)
+template <typename KERN>
+inline int tiled_per_cu(KERN kern, int threads, size_t lds)
+{
+    if (lds > 65536)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds);
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, threads, lds) != hipSuccess || nb < 1) nb = 1;
+    return nb;
+}
+
+// kernel arguments of one grid; slots = resident workgroups available to it
+template <typename TS, typename TC, int R, int VW, int LX, int BY, int MODE>
+inline int fill_tiled_args(const hsx_grid *g, void *out, const void *prn_in, const void *u, const void *coef,
+                           int64_t slots, TiledArgs<TC> &a)
+{
     a.out = out;
+    a.prn_in = prn_in ? prn_in : out;
     a.u = u;
     a.coef = coef;
     a.n1 = g->n1;
     a.n2 = g->n2;
     a.n12 = g->n1 * g->n2;
+    if (a.n12 * (int64_t)sizeof(TS) >= ((int64_t)1 << 32))
+        return fail(HSX_ERR_UNSUPPORTED, "tiled kernels: one n3 plane must stay below 4 GiB");
     a.s1 = g->inner[0];
     a.e1 = g->inner[1];
     a.s2 = g->inner[2];
@@ -406,29 +519,50 @@ inline int launch_tiled(const hsx_grid *g, void *out, const void *u, const void 
     a.nty = (int)((a.e2 - a.s2 + 1 + BY - 1) / BY);
     const int64_t N3 = a.e3 - a.s3 + 1;
     const int64_t tiles = (int64_t)a.ntx * a.nty;
-    const int64_t len = chunk > 0 ? (chunk < N3 ? chunk : N3) : default_chunk(N3, R, tiles);
+    const int64_t len = g_tile.chunk > 0 ? std::min<int64_t>(g_tile.chunk, N3)
+                        : BY == 16        ? chunk_one_round(N3, R, tiles, slots)
+                                          : chunk_short(N3, R, tiles);
     a.chunk = (int)len;
     a.nchunk = (int)((N3 + len - 1) / len);
-    a.nt = g_tile.streams == STREAMS_NT;
+    const int streams = g_tile.flags & (HSX_TILE_CACHED | HSX_TILE_NT);
+    a.nt = streams == HSX_TILE_NT || (streams == 0 && BY == 16);
     a.c = make_coefs<TC>(g);
-    const int64_t nblk = tiles * a.nchunk;
-    if (nblk > (int64_t)1 << 30) return fail(HSX_ERR_UNSUPPORTED, "too many tiles");
+    if (tiles * a.nchunk > (int64_t)1 << 30) return fail(HSX_ERR_UNSUPPORTED, "too many tiles");
+    return HSX_OK;
+}
+
+template <typename TS, typename TC, int R, int VW, int LX, int BY, int MODE, bool RING>
+inline int launch_tiled(const hsx_grid *g, void *out, const void *prn_in, const void *u, const void *coef, hipStream_t s)
+{
+    using K = Tiled<TS, TC, R, VW, LX, BY, MODE, RING>;
+    static_assert(K::LDS <= TILE_LDS_MAX, "LDS tile above the workgroup limit");
+    auto kern = k_tiled<TS, TC, R, VW, LX, BY, MODE, RING>;
+    static const int per_cu = tiled_per_cu(kern, LX * BY, K::LDS);
+    TiledArgs<TC> a;
+    int rc = fill_tiled_args<TS, TC, R, VW, LX, BY, MODE>(g, out, prn_in, u, coef, (int64_t)per_cu * device_cus(), a);
+    if (rc) return rc;
+    const int64_t nblk = (int64_t)a.ntx * a.nty * a.nchunk;
     const unsigned grid = (unsigned)(((nblk + 7) / 8) * 8);
-    static_assert(K::LDS <= TILE_LDS_MAX, "LDS tile above 64 KiB");
-    hipLaunchKernelGGL((k_tiled<TS, TC, R, VW, LX, BY, MODE>), dim3(grid), dim3(LX, BY), K::LDS, s, a);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(LX, BY), K::LDS, s, a);
     HSX_LAUNCHED("k_tiled");
     return HSX_OK;
 }
 
-// BY = 16 where its LDS planes fit (and the tile is at least 32 lanes wide), else 8
+// BY = 16 (with the halo ring where it fits) for tiles of at least 32 lanes, else 8
 template <typename TS, typename TC, int R, int VW, int LX, int MODE>
-inline int launch_tiled_by(const hsx_grid *g, void *out, const void *u, const void *coef, hipStream_t s)
+inline int launch_tiled_by(const hsx_grid *g, void *out, const void *prn_in, const void *u, const void *coef, hipStream_t s)
 {
-    constexpr bool tall_ok = LX >= 32 && Tiled<TS, TC, R, VW, LX, 16, MODE>::LDS <= TILE_LDS_MAX;
-    if constexpr (tall_ok) {
-        if (g_tile.by != 8) return launch_tiled<TS, TC, R, VW, LX, 16, MODE>(g, out, u, coef, g_tile.chunk, s);
+    constexpr bool ring_ok = LX >= 32 && Tiled<TS, TC, R, VW, LX, 16, MODE, true>::RING &&
+                             Tiled<TS, TC, R, VW, LX, 16, MODE, true>::LDS <= TILE_LDS_MAX;
+    constexpr bool tall_ok = LX >= 32 && Tiled<TS, TC, R, VW, LX, 16, MODE, false>::LDS <= TILE_LDS_MAX;
+    if (g_tile.by != 8) {
+        if constexpr (ring_ok) {
+            if (!(g_tile.flags & HSX_TILE_NO_RING))
+                return launch_tiled<TS, TC, R, VW, LX, 16, MODE, true>(g, out, prn_in, u, coef, s);
+        }
+        if constexpr (tall_ok) return launch_tiled<TS, TC, R, VW, LX, 16, MODE, false>(g, out, prn_in, u, coef, s);
     }
-    return launch_tiled<TS, TC, R, VW, LX, 8, MODE>(g, out, u, coef, g_tile.chunk, s);
+    return launch_tiled<TS, TC, R, VW, LX, 8, MODE, false>(g, out, prn_in, u, coef, s);
 }
 
 // alignment and size conditions of the tiled kernels
@@ -441,7 +575,7 @@ inline int tiled_fits(const hsx_grid *g, int vw_elems, size_t elem)
 }
 
 template <int MODE>
-inline int run_tiled(const hsx_grid *g, void *out, const void *u, const void *coef, hipStream_t s)
+inline int run_tiled(const hsx_grid *g, void *out, const void *prn_in, const void *u, const void *coef, hipStream_t s)
 {
     const int R = g->fd_order / 2;
     return dispatch_types(g, [&](auto ts, auto tc) -> int {
@@ -450,14 +584,14 @@ inline int run_tiled(const hsx_grid *g, void *out, const void *u, const void *co
         constexpr int VW = 16 / sizeof(TS);
         int rc = tiled_fits(g, VW, sizeof(TS));
         if (rc) return rc;
-        if (((uintptr_t)out | (uintptr_t)u | (uintptr_t)coef) % 16)
+        if (((uintptr_t)out | (uintptr_t)prn_in | (uintptr_t)u | (uintptr_t)coef) % 16)
             return fail(HSX_ERR_UNSUPPORTED, "buffers not 16-byte aligned");
         const int lx = pick_lx(g, VW);
         return dispatch_R(R, [&](auto rc_) -> int {
             constexpr int RR = decltype(rc_)::value;
-            if (lx == 16) return launch_tiled<TS, TC, RR, VW, 16, 8, MODE>(g, out, u, coef, g_tile.chunk, s);
-            if (lx == 32) return launch_tiled_by<TS, TC, RR, VW, 32, MODE>(g, out, u, coef, s);
-            return launch_tiled_by<TS, TC, RR, VW, 64, MODE>(g, out, u, coef, s);
+            if (lx == 16) return launch_tiled<TS, TC, RR, VW, 16, 8, MODE, false>(g, out, prn_in, u, coef, s);
+            if (lx == 32) return launch_tiled_by<TS, TC, RR, VW, 32, MODE>(g, out, prn_in, u, coef, s);
+            return launch_tiled_by<TS, TC, RR, VW, 64, MODE>(g, out, prn_in, u, coef, s);
         });
     });
 }

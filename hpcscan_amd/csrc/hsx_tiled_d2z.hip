@@ -5,7 +5,7 @@ namespace hsx {
 
 int tiled_fd_d2_z(const hsx_grid *g, const void *u, void *w, hipStream_t s)
 {
-    return run_tiled<MODE_D2Z>(g, w, u, nullptr, s);
+    return run_tiled<MODE_D2Z>(g, w, nullptr, u, nullptr, s);
 }
 
 } // namespace hsx

@@ -97,15 +97,18 @@ int         hsx_set_kernel(int family);
 int         hsx_get_kernel(void);
 /* Tile shape of the tiled kernels (0 = the library's choice for each):
  *   lanes_n1      16, 32 or 64 lanes of 16 bytes along n1 (default by grid width)
- *   rows_n2       8 or 16 rows along n2 (16 only where two LDS planes fit in
- *                 64 KiB and lanes_n1 >= 32; otherwise 8 is used)
+ *   rows_n2       8 or 16 rows along n2 (16 needs lanes_n1 >= 32 and its LDS
+ *                 planes within the 160 KiB of a workgroup; otherwise 8)
  *   chunk_planes  n3 planes streamed by one workgroup
- *   streams       HSX_STREAMS_CACHED or HSX_STREAMS_NT: how prn, coef and the
- *                 output are accessed (nontemporal keeps them out of the caches)
+ *   flags         HSX_TILE_CACHED or HSX_TILE_NT: how prn, coef and the output
+ *                 are accessed (nontemporal keeps them out of the caches);
+ *                 HSX_TILE_NO_RING: load each plane's halo one plane ahead
+ *                 instead of with the queue front (R+1 planes ahead)
  * Process-wide; results do not depend on it. */
-#define HSX_STREAMS_CACHED 1
-#define HSX_STREAMS_NT     2
-int         hsx_set_tile(int lanes_n1, int rows_n2, int chunk_planes, int streams);
+#define HSX_TILE_CACHED  1
+#define HSX_TILE_NT      2
+#define HSX_TILE_NO_RING 4
+int         hsx_set_tile(int lanes_n1, int rows_n2, int chunk_planes, int flags);
 
 /* ---- grid filling: Grid::fill (src/grid.cpp:1650-1826) -------------------- */
 /* u[range] = val (converted to the storage type) */
@@ -138,6 +141,11 @@ int hsx_update_pressure(const hsx_grid *g, void *prn, const void *prc, const voi
                         const void *lapla, void *stream);
 /* Grid::computePressureWithFD (src/grid.cpp:4612-5112): prn = 2 prc - prn + coef*Lap(prc) */
 int hsx_ac2_step(const hsx_grid *g, void *prn, const void *prc, const void *coef, void *stream);
+/* the same step out of place: out = 2 prc - prn + coef*Lap(prc) (out may be prn,
+ * neither may be prc); used where the old wavefield must be kept (the adjoint's
+ * recomputation of a checkpoint segment).  No counterpart in the reference. */
+int hsx_ac2_step_to(const hsx_grid *g, void *out, const void *prn, const void *prc, const void *coef,
+                    void *stream);
 
 /* ---- reductions: Grid::getSumAbs ... (src/grid.cpp:2006-2370) -------------
  * All accumulate in double on the device; the result is returned on the host. */
@@ -153,6 +161,81 @@ int hsx_max_err(const hsx_grid *g, const void *u, const void *v, const int64_t r
 /* out[0] = min(FLT_MAX, u...), out[1] = max(-FLT_MAX, u...) in the storage type */
 int hsx_min_max(const hsx_grid *g, const void *u, const int64_t range[6], double out[2],
                 void *stream);
+
+/* ---- point sources and receivers (DataAcquisition, src/data_Acquisition.cpp:58-305) ----
+ * idx, val, w and out are DEVICE arrays; idx holds flat grid indices, distinct
+ * within one call; values are in the storage type. */
+#define HSX_INJECT_ADD 0   /* u[idx[i]] += scale * val[i] * w[i]   (w may be NULL: 1) */
+#define HSX_INJECT_SET 1   /* u[idx[i]]  = scale * val[i] * w[i]   (the reference's source term) */
+int hsx_inject(const hsx_grid *g, void *u, int64_t n, const int64_t *idx, const void *val, const void *w,
+               double scale, int mode, void *stream);
+/* out[i] = u[idx[i]] (DataAcquisition::recordTrace) */
+int hsx_gather(const hsx_grid *g, const void *u, int64_t n, const int64_t *idx, void *out, void *stream);
+
+/* ---- adjoint of the Ac2 propagator (no counterpart in the reference: its
+ * ADJOINT mode is a stub, src/data_Acquisition.cpp:307-310) ----------------
+ * Imaging condition, accumulated in double over range:
+ *   grad += scale * mu * (u_next - 2 u + u_prev - f),  f = src_val at src_idx. */
+int hsx_imaging(const hsx_grid *g, double *grad, const void *mu, const void *u_next, const void *u,
+                const void *u_prev, const int64_t range[6], double scale, int64_t src_idx, double src_val,
+                void *stream);
+
+/* ---- low-precision checkpoints of FP32 wavefields ----------------------------
+ * A snapshot of n values takes hsx_ckpt_bytes(n) bytes: a 256-byte header
+ * (scale, max |value|) and n 16-bit codes, code = round(v / scale) with
+ * scale = max|v| / 16384 (FP16) or max|v| (BF16).  With u_prev non-NULL the
+ * snapshot holds u - u_prev; decoding with base non-NULL gives base - value. */
+#define HSX_CKPT_FP32 0   /* checkpoints kept in the storage type (bit-exact recomputation) */
+#define HSX_CKPT_FP16 1
+#define HSX_CKPT_BF16 2
+size_t hsx_ckpt_bytes(int64_t n);
+int hsx_ckpt_encode(const float *u, const float *u_prev, int64_t n, void *packed, int format, void *stream);
+int hsx_ckpt_decode(const void *packed, int64_t n, int format, const float *base, float *out, void *stream);
+
+/* ---- one shot: forward modeling and the misfit gradient -----------------------
+ * Step n (0 <= n < nt) is the propagator step (free surface, fused Ac2 step) of
+ * q^{n+1} from q^n, q^{n-1} (q^0 = q^-1 = 0), then wavelet[n*nsrc + i] is added
+ * at src_idx[i], then traces[n*nrec + r] = q^{n+1}[rec_idx[r]].  All arrays
+ * DEVICE, values in the storage type. */
+typedef struct {
+    int64_t nt;
+    double dt;                 /* time step (scales the imaging condition) */
+    int64_t nsrc;
+    const int64_t *src_idx;
+    const void *wavelet;       /* nt x nsrc */
+    int64_t nrec;
+    const int64_t *rec_idx;
+} hsx_shot;
+
+int hsx_forward(const hsx_grid *g, const void *coef, const hsx_shot *shot, void *traces, void *stream);
+
+typedef struct {
+    int ckpt_every;            /* steps per checkpoint segment; 0 = keep every wavefield */
+    int ckpt_format;           /* HSX_CKPT_FP32 / FP16 / BF16 (16-bit: FP32 grids only) */
+} hsx_adjoint_opts;
+
+/* grad (DEVICE, double, npoint values) += dJ/dm for J = 1/2 sum (traces - d_obs)^2,
+ * m = dt^2 / coef the squared slowness, over the inner points not on a
+ * boundary plane; *misfit = J (may be NULL).  Forward pass with checkpoints,
+ * then the adjoint backwards, recomputing one segment at a time.  Allocates its
+ * work buffers and returns after the stream has finished with them. */
+int hsx_gradient(const hsx_grid *g, const void *coef, const hsx_shot *shot, const void *d_obs,
+                 const hsx_adjoint_opts *opts, double *grad, double *misfit, void *stream);
+
+/* ---- batched shots (no counterpart in the reference: one grid per process) ----
+ * nshots forward modelings (hsx_forward of each shot, same results) advanced
+ * together: per time step one launch of the free surface pair, one fused-step
+ * launch over the tiles of every grid and one source/receiver launch.  Grids:
+ * 3D, one FD order and precision (storage = compute), any sizes, n1 a multiple
+ * of 16 bytes; every shot the same nt.  coef[s] and traces[s] are DEVICE
+ * buffers kept alive by the caller; the batch owns the wavefields. */
+typedef struct hsx_batch hsx_batch;
+int hsx_batch_create(int nshots, const hsx_grid *grids, const void *const *coef, const hsx_shot *shots,
+                     void *const *traces, hsx_batch **out);
+/* all nt steps from zero wavefields; use_graph: capture the time loop into a
+ * HIP graph on the first call, replay it on every call */
+int hsx_batch_run(hsx_batch *b, int use_graph, void *stream);
+int hsx_batch_destroy(hsx_batch *b);
 
 #ifdef __cplusplus
 }

@@ -109,7 +109,14 @@ double orc_max_err_##SFX(const orc_grid *g, const TS *u, const TS *v,           
 double orc_l1_err_##SFX(const orc_grid *g, const TS *u, const TS *v,                    \
                         const int64_t r[6]);                                            \
 double orc_min_##SFX(const orc_grid *g, const TS *u, const int64_t r[6]);               \
-double orc_max_##SFX(const orc_grid *g, const TS *u, const int64_t r[6]);
+double orc_max_##SFX(const orc_grid *g, const TS *u, const int64_t r[6]);              \
+int    orc_forward_##SFX(const orc_grid *g, const TS *coef, const TC inv2[3], int64_t nt,   \
+                         int64_t nsrc, const int64_t *src_idx, const TS *wavelet,          \
+                         int64_t nrec, const int64_t *rec_idx, TS *traces);                \
+int    orc_gradient_##SFX(const orc_grid *g, const TS *coef, const TC inv2[3], double dt,   \
+                          int64_t nt, int64_t nsrc, const int64_t *src_idx,                \
+                          const TS *wavelet, int64_t nrec, const int64_t *rec_idx,         \
+                          const TS *d_obs, double *grad, double *misfit);
 
 ORC_DECLARE(f32, float, float)
 ORC_DECLARE(f64, double, double)

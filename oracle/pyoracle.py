@@ -105,6 +105,10 @@ def lib():
             getattr(L, f"orc_laplacian_{sfx}").argtypes = [P(OrcGrid), vp, vp, vp, OrcSched]
             getattr(L, f"orc_update_pressure_{sfx}").argtypes = [P(OrcGrid), vp, vp, vp, vp]
             getattr(L, f"orc_pressure_fd_{sfx}").argtypes = [P(OrcGrid), vp, vp, vp, vp, OrcSched]
+            i64 = ctypes.c_int64
+            getattr(L, f"orc_forward_{sfx}").argtypes = [P(OrcGrid), vp, vp, i64, i64, vp, vp, i64, vp, vp]
+            getattr(L, f"orc_gradient_{sfx}").argtypes = [P(OrcGrid), vp, vp, ctypes.c_double, i64, i64, vp,
+                                                          vp, i64, vp, vp, vp, P(ctypes.c_double)]
             for red in ("sum_abs", "min", "max"):
                 f = getattr(L, f"orc_{red}_{sfx}")
                 f.argtypes = [P(OrcGrid), vp, P(ctypes.c_int64)]
@@ -371,6 +375,36 @@ def pressure_fd(g, prn, prc, coef, iv, s=None, compute64=False):
     s = s or sched()
     return getattr(lib(), f"orc_pressure_fd_{_sfx(prn.dtype, compute64)}")(
         ctypes.byref(g), _p(prn), _p(prc), _p(coef), _p(iv), s)
+
+
+def forward(g, coef, iv, nt, src_idx, wavelet, rec_idx, compute64=False):
+    """Shot modeling (propagator step + additive point sources), traces[nt, nrec]."""
+    src_idx = np.ascontiguousarray(src_idx, np.int64)
+    rec_idx = np.ascontiguousarray(rec_idx, np.int64)
+    wavelet = np.ascontiguousarray(wavelet, coef.dtype)
+    traces = np.zeros((nt, len(rec_idx)), coef.dtype)
+    rc = getattr(lib(), f"orc_forward_{_sfx(coef.dtype, compute64)}")(
+        ctypes.byref(g), _p(coef), _p(iv), nt, len(src_idx), _p(src_idx), _p(wavelet), len(rec_idx),
+        _p(rec_idx), _p(traces))
+    if rc:
+        raise MemoryError("oracle forward: allocation failed")
+    return traces
+
+
+def gradient(g, coef, iv, dt, nt, src_idx, wavelet, rec_idx, d_obs, compute64=False):
+    """dJ/dm (double, npoint) and J for J = 1/2 |traces - d_obs|^2, full storage."""
+    src_idx = np.ascontiguousarray(src_idx, np.int64)
+    rec_idx = np.ascontiguousarray(rec_idx, np.int64)
+    wavelet = np.ascontiguousarray(wavelet, coef.dtype)
+    d_obs = np.ascontiguousarray(d_obs, coef.dtype)
+    grad = np.zeros(g.npoint, np.float64)
+    mis = ctypes.c_double(0.0)
+    rc = getattr(lib(), f"orc_gradient_{_sfx(coef.dtype, compute64)}")(
+        ctypes.byref(g), _p(coef), _p(iv), float(dt), nt, len(src_idx), _p(src_idx), _p(wavelet),
+        len(rec_idx), _p(rec_idx), _p(d_obs), _p(grad), ctypes.byref(mis))
+    if rc:
+        raise MemoryError("oracle gradient: allocation failed")
+    return grad, mis.value
 
 
 def reduce(op, g, u, v=None, ptype=INNER_POINTS):

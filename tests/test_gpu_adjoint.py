@@ -1,0 +1,201 @@
+"""Shot modeling and the misfit gradient on the GPU against the CPU oracle:
+traces and gradients bit for bit (full storage and storage-type checkpoints),
+16-bit checkpoints against the FP64 oracle gradient, and the building blocks
+(sources / receivers, imaging condition, checkpoint codec, out-of-place step)."""
+import numpy as np
+import pytest
+
+from hpcscan_amd.layout import make_layout
+from tests.shots import make_shot
+
+pytestmark = pytest.mark.gpu
+
+
+def _same(a, b):
+    assert a.dtype == b.dtype and a.shape == b.shape
+    eq = (a == b) | (np.isnan(a) & np.isnan(b))
+    if not eq.all():
+        i = np.flatnonzero(~eq.ravel())[:5]
+        raise AssertionError(f"{(~eq).sum()} values differ, e.g. at {i}: {a.ravel()[i]} vs {b.ravel()[i]}")
+
+
+def _dev(capi, a):
+    return capi.DeviceArray.from_numpy(np.ascontiguousarray(a))
+
+
+def _shot_dev(capi, sh):
+    d = dict(src=_dev(capi, sh["src"]), rec=_dev(capi, sh["rec"]), wav=_dev(capi, sh["wavelet"]),
+             coef=_dev(capi, sh["coef"]))
+    s = capi.make_shot(sh["nt"], sh["dt"], (d["src"], len(sh["src"])), d["wav"], (d["rec"], len(sh["rec"])))
+    return d, s
+
+
+SIZES = [(20, 18, 16), (68, 40, 24)]   # narrow tiles (8 rows) / 32-lane 16-row tiles with the halo ring
+
+
+@pytest.mark.parametrize("n", SIZES)
+@pytest.mark.parametrize("fp64", [False, True])
+@pytest.mark.parametrize("order", [4, 8])
+def test_forward_traces_bit_exact(oracle, capi, n, fp64, order):
+    print(f"forward n={n} fp64={fp64} order={order}", flush=True)
+    sh = make_shot(oracle, n, order, nt=40, fp64=fp64, seed=order, with_obs=False)
+    g = capi.hsx_grid(sh["p"])
+    d, s = _shot_dev(capi, sh)
+    tr = capi.DeviceArray(sh["nt"] * len(sh["rec"]) * np.dtype(sh["dtype"]).itemsize)
+    capi.forward(g, d["coef"], s, tr)
+    ref = oracle.forward(sh["o"], sh["coef"], sh["iv"], sh["nt"], sh["src"], sh["wavelet"], sh["rec"])
+    assert np.abs(ref).max() > 0
+    _same(tr.download(sh["dtype"]).reshape(ref.shape), ref)
+
+
+@pytest.mark.parametrize("fp64,every", [(False, 0), (False, 7), (True, 0), (True, 9), (False, 45)])
+def test_gradient_bit_exact(oracle, capi, fp64, every):
+    """full storage and storage-type checkpoints give the oracle's gradient bit for bit"""
+    print(f"gradient fp64={fp64} every={every}", flush=True)
+    sh = make_shot(oracle, (24, 20, 18), 8, nt=45, fp64=fp64, seed=11)
+    g = capi.hsx_grid(sh["p"])
+    d, s = _shot_dev(capi, sh)
+    dobs = _dev(capi, sh["d_obs"])
+    grad = _dev(capi, np.zeros(sh["p"].npoint))
+    J = capi.gradient(g, d["coef"], s, dobs, grad, every, capi.HSX_CKPT_FP32)
+    gref, Jref = oracle.gradient(sh["o"], sh["coef"], sh["iv"], sh["dt"], sh["nt"], sh["src"], sh["wavelet"],
+                                 sh["rec"], sh["d_obs"])
+    assert J == pytest.approx(Jref, rel=1e-12)
+    assert np.abs(gref).max() > 0
+    _same(grad.download(np.float64), gref)
+
+
+@pytest.mark.parametrize("fmt,tol", [("fp16", 1e-3), ("bf16", 1e-2)])
+def test_low_precision_checkpoints_against_fp64_oracle(oracle, capi, fmt, tol):
+    """FP32 stencil, 16-bit scaled checkpoints, FP64 gradient accumulation: the
+    error against the FP64 oracle gradient"""
+    sh64 = make_shot(oracle, (40, 36, 32), 8, nt=120, fp64=True, seed=21)
+    sh = make_shot(oracle, (40, 36, 32), 8, nt=120, fp64=False, seed=21)
+    g64, _ = oracle.gradient(sh64["o"], sh64["coef"], sh64["iv"], sh64["dt"], sh64["nt"], sh64["src"],
+                             sh64["wavelet"], sh64["rec"], sh64["d_obs"])
+    g = capi.hsx_grid(sh["p"])
+    d, s = _shot_dev(capi, sh)
+    dobs = _dev(capi, sh["d_obs"])
+    errs = {}
+    for name, f in (("fp32", capi.HSX_CKPT_FP32), (fmt, {"fp16": capi.HSX_CKPT_FP16, "bf16": capi.HSX_CKPT_BF16}[fmt])):
+        grad = _dev(capi, np.zeros(sh["p"].npoint))
+        capi.gradient(g, d["coef"], s, dobs, grad, 20, f)
+        x = grad.download(np.float64)
+        errs[name] = float(np.linalg.norm(x - g64) / np.linalg.norm(g64))
+    print(f"relative L2 error vs FP64 oracle: {errs}", flush=True)
+    assert errs["fp32"] < 1e-4
+    assert errs[fmt] < tol
+
+
+@pytest.mark.parametrize("fmt", ["fp16", "bf16"])
+def test_checkpoint_codec(capi, fmt):
+    f = {"fp16": capi.HSX_CKPT_FP16, "bf16": capi.HSX_CKPT_BF16}[fmt]
+    ulp = {"fp16": 2.0 ** -11, "bf16": 2.0 ** -8}[fmt]
+    rng = np.random.default_rng(4)
+    n = 100003
+    u = (rng.standard_normal(n) * 3e-3).astype(np.float32)
+    up = (u - rng.standard_normal(n).astype(np.float32) * 1e-6).astype(np.float32)
+    du, dup = _dev(capi, u), _dev(capi, up)
+    packed = capi.DeviceArray(capi.ckpt_bytes(n))
+    out = capi.DeviceArray(n * 4)
+    capi.ckpt_encode(du, None, n, packed, f)
+    capi.ckpt_decode(packed, n, f, None, out)
+    x = out.download(np.float32)
+    big = np.abs(u) > np.abs(u).max() * 2.0 ** -13
+    assert np.all(np.abs(x - u)[big] <= ulp * np.abs(u)[big] * 1.0001)
+    # pair: u - u_prev keeps its own scale; decoding with base u gives u_prev back
+    capi.ckpt_encode(du, dup, n, packed, f)
+    capi.ckpt_decode(packed, n, f, du, out)
+    y = out.download(np.float32)
+    dd = (u - up).astype(np.float64)
+    assert np.max(np.abs(y - up)) <= ulp * np.abs(dd).max() * 1.01 + 2 * np.spacing(np.abs(u).max())
+    z = np.zeros(n, np.float32)
+    capi.ckpt_encode(_dev(capi, z), None, n, packed, f)
+    capi.ckpt_decode(packed, n, f, None, out)
+    assert np.all(out.download(np.float32) == 0)
+
+
+@pytest.mark.parametrize("mode", ["add", "set"])
+def test_inject_and_gather(capi, mode):
+    p = make_layout(3, (12, 11, 10), 4)
+    g = capi.hsx_grid(p)
+    rng = np.random.default_rng(2)
+    u = rng.standard_normal(p.npoint).astype(np.float32)
+    idx = rng.choice(p.npoint, 37, replace=False).astype(np.int64)
+    val = rng.standard_normal(37).astype(np.float32)
+    w = rng.standard_normal(37).astype(np.float32)
+    du, di, dv, dw = (_dev(capi, x) for x in (u, idx, val, w))
+    m = capi.HSX_INJECT_ADD if mode == "add" else capi.HSX_INJECT_SET
+    capi.inject(g, du, 37, di, dv, dw, 0.5, m)
+    ref = u.copy()
+    v = 0.5 * val.astype(np.float64) * w.astype(np.float64)
+    ref[idx] = (ref[idx].astype(np.float64) + v).astype(np.float32) if mode == "add" else v.astype(np.float32)
+    got = du.download(np.float32)
+    _same(got, ref)
+    out = capi.DeviceArray(37 * 4)
+    capi.gather(g, du, 37, di, out)
+    _same(out.download(np.float32), got[idx])
+
+
+def test_imaging_condition(capi):
+    p = make_layout(3, (13, 12, 11), 8)
+    g = capi.hsx_grid(p)
+    rng = np.random.default_rng(3)
+    mu, un, u, up = (rng.standard_normal(p.npoint).astype(np.float32) for _ in range(4))
+    grad0 = rng.standard_normal(p.npoint)
+    r = p.inner()
+    rim = [r[0] + 1, r[1] - 1, r[2] + 1, r[3] - 1, r[4] + 1, r[5] - 1]
+    src = rim[0] + 2 + (rim[2] + 3) * p.n1 + (rim[4] + 1) * p.n1 * p.n2
+    dg = _dev(capi, grad0)
+    capi.imaging(g, dg, *(_dev(capi, x) for x in (mu, un, u, up)), rim, -3.5, src, 0.25)
+    ref = grad0.copy()
+    box = np.zeros((p.n3, p.n2, p.n1), bool)
+    box[rim[4]:rim[5] + 1, rim[2]:rim[3] + 1, rim[0]:rim[1] + 1] = True
+    box = box.ravel()
+    d2 = (un.astype(np.float64) - 2.0 * u.astype(np.float64)) + up.astype(np.float64)
+    d2[src] -= 0.25
+    ref[box] += -3.5 * mu.astype(np.float64)[box] * d2[box]
+    _same(dg.download(np.float64), ref)
+
+
+def test_out_of_place_step(oracle, capi):
+    p = make_layout(3, (36, 20, 18), 8)
+    g = capi.hsx_grid(p)
+    rng = np.random.default_rng(9)
+    prn, prc, coef = (rng.standard_normal(p.npoint).astype(np.float32) for _ in range(3))
+    dprn, dprc, dcoef = (_dev(capi, x) for x in (prn, prc, coef))
+    out = capi.DeviceArray(p.npoint * 4)
+    capi.ac2_step_to(g, out, dprn, dprc, dcoef)
+    capi.ac2_step(g, dprn, dprc, dcoef)
+    _same(out.download(np.float32)[np.array(masks_inner(p))], dprn.download(np.float32)[np.array(masks_inner(p))])
+    with pytest.raises(capi.HsxError):
+        capi.ac2_step_to(g, dprc, dprn, dprc, dcoef)
+
+
+def masks_inner(p):
+    m = np.zeros((p.n3, p.n2, p.n1), bool)
+    r = p.inner()
+    m[r[4]:r[5] + 1, r[2]:r[3] + 1, r[0]:r[1] + 1] = True
+    return m.ravel()
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batched_shots_equal_single_shots(oracle, capi, fp64, use_graph):
+    """uneven grids in one batch (one fused-step launch per time step, optionally
+    a captured graph replayed twice): every shot's traces equal the oracle's"""
+    sizes = [(36, 20, 18), (40, 24, 22), (28, 30, 26), (68, 36, 20)]
+    shots, keep = [], []
+    for k, n in enumerate(sizes):
+        sh = make_shot(oracle, n, 8, nt=30, fp64=fp64, seed=50 + k, with_obs=False)
+        d, s = _shot_dev(capi, sh)
+        tr = capi.DeviceArray(sh["nt"] * len(sh["rec"]) * np.dtype(sh["dtype"]).itemsize)
+        shots.append((sh, d, s, tr))
+    b = capi.Batch([capi.hsx_grid(x[0]["p"]) for x in shots], [x[1]["coef"] for x in shots],
+                   [x[2] for x in shots], [x[3] for x in shots])
+    for rep in range(2 if use_graph else 1):
+        b.run(use_graph=use_graph)
+        for sh, d, s, tr in shots:
+            ref = oracle.forward(sh["o"], sh["coef"], sh["iv"], sh["nt"], sh["src"], sh["wavelet"], sh["rec"])
+            _same(tr.download(sh["dtype"]).reshape(ref.shape), ref)
+    b.close()

@@ -306,35 +306,41 @@ def _n1_aligned(target, order, dtype, vw):
 
 
 TILED_CASES = [
-    # (order, prec, n1 target, n2, n3, lanes, chunk, rows, streams)
+    # (order, prec, n1 target, n2, n3, lanes, chunk, rows, flags); flags 1 cached, 2 nontemporal, 4 no ring
     (8, "f32", 72, 33, 29, 16, 5, 0, 0),      # several tiles in n1 and n2, chunk < 2R+1
     (8, "f32", 72, 33, 29, 16, 9, 0, 2),      # chunk == 2R+1, nontemporal streams
     (8, "f32", 130, 17, 40, 32, 13, 8, 1),
-    (8, "f32", 130, 37, 40, 32, 13, 16, 2),   # 16 rows, partial last tile row
+    (8, "f32", 130, 37, 40, 32, 13, 16, 2),   # 16 rows with the halo ring, partial last tile row
+    (8, "f32", 130, 37, 40, 32, 13, 16, 6),   # 16 rows without the ring
     (8, "f32", 200, 24, 31, 64, 0, 8, 0),     # one wide tile, default chunk
     (8, "f32", 200, 40, 31, 64, 7, 16, 1),
+    (8, "f32", 200, 40, 3, 64, 0, 16, 0),     # fewer planes than R: ring filled past the chunk
     (2, "f32", 69, 20, 22, 16, 7, 0, 0),      # odd halo: n1 offset, tile origin below the inner start
+    (2, "f32", 140, 20, 22, 32, 5, 16, 0),    # R = 1 ring (two slots)
     (6, "f32", 69, 19, 23, 16, 4, 0, 0),
     (6, "f32", 140, 35, 23, 32, 4, 16, 2),
     (16, "f32", 68, 25, 37, 16, 6, 0, 0),     # R = 8: two n2 halo rows per thread at 8 rows
     (16, "f32", 140, 9, 20, 64, 0, 8, 0),
-    (16, "f32", 140, 41, 20, 32, 0, 16, 2),   # R = 8 at 16 rows
+    (16, "f32", 140, 41, 20, 32, 0, 16, 2),   # R = 8 at 16 rows with the ring
+    (16, "f32", 270, 41, 20, 64, 0, 16, 0),   # R = 8, 64 lanes: ring too large, 16 rows without it
     (4, "f64", 70, 21, 27, 16, 5, 0, 0),      # VW = 2
     (4, "f64", 150, 37, 27, 64, 9, 16, 2),
+    (4, "f64", 150, 37, 27, 64, 9, 16, 4),
     (12, "f64", 66, 30, 26, 32, 11, 0, 1),
     (8, "f32c64", 72, 18, 21, 16, 5, 0, 0),   # FP32 storage, FP64 compute
     (8, "f32c64", 140, 35, 21, 64, 5, 16, 2),
 ]
 
 
-@pytest.mark.parametrize("order,prec,n1,n2,n3,lanes,chunk,rows,streams", TILED_CASES)
-def test_tiled_shapes_bit_exact(oracle, capi, order, prec, n1, n2, n3, lanes, chunk, rows, streams):
+@pytest.mark.parametrize("order,prec,n1,n2,n3,lanes,chunk,rows,flags", TILED_CASES)
+def test_tiled_shapes_bit_exact(oracle, capi, order, prec, n1, n2, n3, lanes, chunk, rows, flags):
     """The LDS-tiled kernels, forced, on partial tiles and chunk boundaries."""
     dtype, c64 = PRECS[prec]
     vw = 16 // np.dtype(dtype).itemsize
     n = (_n1_aligned(n1, order, dtype, vw), n2, n3)
+    streams = flags
     print(f"tiled order={order} prec={prec} n={n} lanes={lanes} chunk={chunk} rows={rows} "
-          f"streams={streams}", flush=True)
+          f"flags={flags}", flush=True)
     p, o = _layouts(oracle, 3, n, order, dtype)
     assert p.n1 % vw == 0
     rng = np.random.default_rng(order * 7 + n3)
@@ -365,11 +371,11 @@ def test_tiled_shapes_bit_exact(oracle, capi, order, prec, n1, n2, n3, lanes, ch
         capi.set_kernel(capi.HSX_KERNEL_AUTO)
 
 
-@pytest.mark.parametrize("args", [(8, 0, 0, 0), (64, -1, 0, 0), (64, 0, 12, 0), (64, 0, 0, 3)])
+@pytest.mark.parametrize("args", [(8, 0, 0, 0), (64, -1, 0, 0), (64, 0, 12, 0), (64, 0, 0, 3), (64, 0, 0, 8)])
 def test_set_tile_rejects_bad_shapes(capi, args):
-    lanes, chunk, rows, streams = args
+    lanes, chunk, rows, flags = args
     with pytest.raises(capi.HsxError):
-        capi.set_tile(lanes, chunk, rows, streams)
+        capi.set_tile(lanes, chunk, rows, flags)
 
 
 def test_tiled_rejects_misaligned_rows(capi):
