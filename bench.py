@@ -11,6 +11,10 @@ are resident in HBM before timing.  Prints one JSON line.
              (read prc, prn, coef; write prn) / its average HIP-event duration
   cpu_baseline  the oracle's CacheBlk restatement on this host's cores, on a
              time-bounded sample of the same workload
+  adjoint    the second half of the metric: seconds per shot of the misfit
+             gradient (200^3 order 8, 1000 steps, 16-bit checkpoints) against
+             the plain PyTorch full-storage version, with both gradients'
+             error against the FP64 gradient (see adjoint_leg)
 """
 import argparse
 import json
@@ -200,6 +204,119 @@ def cpu_leg(args):
                       f"oracle CacheBlk cb 9999/4/16, {cores} OpenMP threads"}
 
 
+def adjoint_leg(args):
+    """Misfit gradient of one shot: n^3 FP32 order 8, smooth velocity 2000-4000 m/s,
+    h = 10 m, 8 Hz Ricker one point below the top centre, 200 receivers on a line
+    one point below the top, observed data from a +-3 % smoothly perturbed model,
+    nt steps at 0.8 x CFL.  Times (s per shot, whole gradient incl. forward):
+      ours   hsx_gradient, checkpoints every K steps in the chosen 16-bit format
+      plain  the same scheme in PyTorch tensor ops, FP32, autograd (full storage)
+    Errors: relative L2 against the FP64 gradient (hsx_gradient on FP64 grids
+    with storage-type checkpoints, i.e. exact FP64 recomputation)."""
+    import numpy as np
+    import torch
+
+    from hpcscan_amd import acquisition as acq
+    from hpcscan_amd import capi
+    from hpcscan_amd.layout import make_layout
+    from hpcscan_amd.plain import plain_forward
+
+    n, nt, K = args.adj_n, args.adj_nt, args.adj_every
+    fmt = {"fp16": capi.HSX_CKPT_FP16, "bf16": capi.HSX_CKPT_BF16, "fp32": capi.HSX_CKPT_FP32}[args.adj_format]
+    h, vmin, vmax, f0 = 10.0, 2000.0, 4000.0, 8.0
+    lay = {}
+    for fp64 in (False, True):
+        p = make_layout(3, (n, n, n), 8, fp64=fp64)
+        p.d = [h, h, h]
+        lay[fp64] = p
+    p = lay[False]
+    v = acq.smooth_velocity(p, vmin, vmax, seed=4)
+    m = 1.0 / (v * v)
+    m_true = m * (1.0 + 0.03 * np.clip(acq.gaussian_field((p.n3, p.n2, p.n1), 44, 16.0).ravel() / 3.0, -1, 1))
+    dt = float(np.float32(0.8 * acq.fd_cfl(8) * h / vmax))
+    r = p.inner()
+    inner = np.zeros((p.n3, p.n2, p.n1), bool)
+    inner[r[4]:r[5] + 1, r[2]:r[3] + 1, r[0]:r[1] + 1] = True
+    inner = inner.ravel()
+    src = np.array([acq.source_top_center(p, 1)], np.int64)
+    rec = acq.receiver_line(p, 200, 1)
+    wav = acq.ricker(nt, dt, f0)
+    log(f"adjoint: {n}^3 O8, nt {nt}, dt {dt:.3e} s, {len(rec)} receivers, checkpoints every {K} ({args.adj_format})")
+
+    def dev(a):
+        return capi.DeviceArray.from_numpy(np.ascontiguousarray(a))
+
+    def setup(fp64):
+        dtp = np.float64 if fp64 else np.float32
+        g = capi.hsx_grid(lay[fp64])
+        coef = dev(np.where(inner, dt * dt / m, 0.0).astype(dtp))
+        coef_t = dev(np.where(inner, dt * dt / m_true, 0.0).astype(dtp))
+        ds, dr, dw = dev(src), dev(rec), dev(wav.astype(dtp).reshape(nt, 1))
+        shot = capi.make_shot(nt, dt, (ds, 1), dw, (dr, len(rec)))
+        return dict(g=g, coef=coef, coef_t=coef_t, keep=(ds, dr, dw), shot=shot, dtype=dtp)
+
+    S64, S32 = setup(True), setup(False)
+    # observed data: FP64 forward of the perturbed model (FP32 runs use it rounded)
+    d64 = capi.DeviceArray(nt * len(rec) * 8)
+    capi.forward(S64["g"], S64["coef_t"], S64["shot"], d64)
+    dobs64 = d64.download(np.float64).reshape(nt, len(rec))
+    dobs32 = dev(dobs64.astype(np.float32))
+
+    def grad_of(S, dobs, every, f, reps=1):
+        best, gout, J = 1e30, None, None
+        for _ in range(reps):
+            gr = capi.DeviceArray.from_numpy(np.zeros(p.npoint))
+            capi.sync()
+            t0 = time.perf_counter()
+            J = capi.gradient(S["g"], S["coef"], S["shot"], dobs, gr, every, f)
+            best = min(best, time.perf_counter() - t0)
+            gout = gr.download(np.float64)
+            gr.free()
+        return best, gout, J
+
+    t64, g64, J64 = grad_of(S64, d64, K, capi.HSX_CKPT_FP32)
+    ours_t, g_ours, J32 = grad_of(S32, dobs32, K, fmt, reps=3)
+    full_t, g_full, _ = grad_of(S32, dobs32, 0, capi.HSX_CKPT_FP32, reps=2)
+    nrm = np.linalg.norm(g64)
+    err_ours = float(np.linalg.norm(g_ours - g64) / nrm)
+    err_full = float(np.linalg.norm(g_full - g64) / nrm)
+    log(f"adjoint: ours {ours_t:.3f} s (err {err_ours:.2e}), full storage FP32 {full_t:.3f} s "
+        f"(err {err_full:.2e}), FP64 {t64:.3f} s")
+
+    # plain PyTorch, FP32, autograd over the whole time loop
+    plain_t, err_plain = None, None
+    if not args.no_plain:
+        mt = torch.tensor(m, dtype=torch.float32, device="cuda", requires_grad=True)
+        dobs_t = torch.tensor(dobs64.astype(np.float32), device="cuda")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tr = plain_forward(p, mt, dt, src, wav.astype(np.float32), rec)
+        loss = 0.5 * ((tr - dobs_t) ** 2).sum()
+        loss.backward()
+        torch.cuda.synchronize()
+        plain_t = time.perf_counter() - t0
+        gp = mt.grad.detach().double().cpu().numpy()
+        err_plain = float(np.linalg.norm(gp - g64) / nrm)
+        del mt, tr, loss
+        torch.cuda.empty_cache()
+        log(f"adjoint: plain PyTorch {plain_t:.3f} s (err {err_plain:.2e})")
+    return {
+        "workload": f"one shot misfit gradient, {n}^3 FP32 FD order 8, {nt} steps, h 10 m, smooth velocity "
+                    f"2000-4000 m/s, 8 Hz Ricker top centre, {len(rec)} receivers, data from a +-3% perturbed model",
+        "s_per_shot": round(ours_t, 4),
+        "checkpoints": f"every {K} steps, {args.adj_format} with per-snapshot max-abs scale, FP64 gradient",
+        "grad_rel_l2_vs_fp64": err_ours,
+        "plain_s_per_shot": None if plain_t is None else round(plain_t, 4),
+        "plain": "same scheme in PyTorch tensor ops, FP32, autograd tape (every step stored)",
+        "plain_grad_rel_l2_vs_fp64": err_plain,
+        "vs_plain": None if plain_t is None else round(plain_t / ours_t, 3),
+        "full_storage_fp32_s_per_shot": round(full_t, 4),
+        "full_storage_fp32_grad_rel_l2_vs_fp64": err_full,
+        "fp64_reference_s": round(t64, 4),
+        "misfit_fp32": J32, "misfit_fp64": J64,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -222,6 +339,12 @@ def main():
                     help="load each plane's halo one plane ahead instead of with the queue front")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one full-grid copy kernel first (counter calibration for profiles)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient leg")
+    ap.add_argument("--no-plain", action="store_true", help="adjoint leg without the plain PyTorch timing")
+    ap.add_argument("--adj-n", type=int, default=200)
+    ap.add_argument("--adj-nt", type=int, default=1000)
+    ap.add_argument("--adj-every", type=int, default=20)
+    ap.add_argument("--adj-format", default="fp16", choices=["fp16", "bf16", "fp32"])
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -233,6 +356,7 @@ def main():
         sys.exit(2)
 
     gpu = gpu_leg(args)
+    adj = None if args.no_adjoint else adjoint_leg(args)
     cpu = None if args.no_cpu_baseline else cpu_leg(args)
     npt = gpu["npt"]
     pts_stencil = 6 * (args.order // 2) + 1
@@ -257,6 +381,7 @@ def main():
                    "streams": args.streams, "halo_ring": not args.no_ring},
         "roofline": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in gpu["roofline"].items()},
         "cpu_baseline": cpu,
+        "adjoint": adj,
         "extra": {"fused_kernel_ms": round(gpu["kernel_ms"], 4),
                   "hpcscan_apparent_GBps": round(gpu["value"] * (pts_stencil + 4) * 4, 1),
                   "device": gpu["device"]["name"], "arch": gpu["device"]["arch"]},

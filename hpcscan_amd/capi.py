@@ -102,6 +102,8 @@ SIGNATURES = {
     "hsx_ckpt_decode": (_i, [_vp, _i64, _i, _vp, _vp, _vp]),
     "hsx_forward": (_i, [_G, _vp, _vp, _vp, _vp]),
     "hsx_gradient": (_i, [_G, _vp, _vp, _vp, _vp, _vp, _P(_d), _vp]),
+    "hsx_autotune": (_i, [_G, _i, _P(_i), _P(ctypes.c_float)]),
+    "hsx_autotune_clear": (_i, []),
     "hsx_batch_create": (_i, [_i, _G, _P(_vp), _vp, _P(_vp), _P(_vp)]),
     "hsx_batch_run": (_i, [_vp, _i, _vp]),
     "hsx_batch_destroy": (_i, [_vp]),
@@ -174,6 +176,18 @@ def set_tile(lanes_n1: int = 0, chunk_planes: int = 0, rows_n2: int = 0, flags: 
     """Tile shape of the tiled kernels; 0 = the library's choice for each.
     flags: HSX_TILE_CACHED or HSX_TILE_NT (streams), | HSX_TILE_NO_RING."""
     check(lib().hsx_set_tile(lanes_n1, rows_n2, chunk_planes, flags))
+
+
+def autotune(g: "HsxGrid", reps: int = 3):
+    """Pick the fastest tile shape for g's layout; returns ((lanes, rows, chunk, flags), ms)."""
+    shape = (ctypes.c_int * 4)()
+    ms = ctypes.c_float(0.0)
+    check(lib().hsx_autotune(ctypes.byref(g), int(reps), shape, ctypes.byref(ms)))
+    return tuple(shape), ms.value
+
+
+def autotune_clear():
+    check(lib().hsx_autotune_clear())
 
 
 def hsx_grid(g: GridLayout, compute_fp64: Optional[bool] = None) -> HsxGrid:

@@ -574,8 +574,12 @@ inline int tiled_fits(const hsx_grid *g, int vw_elems, size_t elem)
     return HSX_OK;
 }
 
+// shape chosen by hsx_autotune for this grid and operation, if any
+bool tuned_shape(const hsx_grid *g, int mode, TileCfg *out);
+
 template <int MODE>
-inline int run_tiled(const hsx_grid *g, void *out, const void *prn_in, const void *u, const void *coef, hipStream_t s)
+inline int run_tiled_shape(const hsx_grid *g, void *out, const void *prn_in, const void *u, const void *coef,
+                           hipStream_t s)
 {
     const int R = g->fd_order / 2;
     return dispatch_types(g, [&](auto ts, auto tc) -> int {
@@ -594,6 +598,20 @@ inline int run_tiled(const hsx_grid *g, void *out, const void *prn_in, const voi
             return launch_tiled_by<TS, TC, RR, VW, 64, MODE>(g, out, prn_in, u, coef, s);
         });
     });
+}
+
+// the forced shape (hsx_set_tile) if any, else the tuned one, else the defaults
+template <int MODE>
+inline int run_tiled(const hsx_grid *g, void *out, const void *prn_in, const void *u, const void *coef, hipStream_t s)
+{
+    const TileCfg forced = g_tile;
+    TileCfg tuned;
+    const bool use_tuned = forced.lx == 0 && forced.by == 0 && forced.chunk == 0 && forced.flags == 0 &&
+                           tuned_shape(g, MODE, &tuned);
+    if (use_tuned) g_tile = tuned;
+    const int rc = run_tiled_shape<MODE>(g, out, prn_in, u, coef, s);
+    if (use_tuned) g_tile = forced;
+    return rc;
 }
 
 } // namespace hsx
