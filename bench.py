@@ -221,19 +221,19 @@ def adjoint_leg(args):
     from hpcscan_amd.layout import make_layout
     from hpcscan_amd.plain import plain_forward
 
-    n, nt, K = args.adj_n, args.adj_nt, args.adj_every
+    n, nt, K, order = args.adj_n, args.adj_nt, args.adj_every, args.adj_order
     fmt = {"fp16": capi.HSX_CKPT_FP16, "bf16": capi.HSX_CKPT_BF16, "fp32": capi.HSX_CKPT_FP32}[args.adj_format]
     h, vmin, vmax, f0 = 10.0, 2000.0, 4000.0, 8.0
     lay = {}
     for fp64 in (False, True):
-        p = make_layout(3, (n, n, n), 8, fp64=fp64)
+        p = make_layout(3, (n, n, n), order, fp64=fp64)
         p.d = [h, h, h]
         lay[fp64] = p
     p = lay[False]
     v = acq.smooth_velocity(p, vmin, vmax, seed=4)
     m = 1.0 / (v * v)
     m_true = m * (1.0 + 0.03 * np.clip(acq.gaussian_field((p.n3, p.n2, p.n1), 44, 16.0).ravel() / 3.0, -1, 1))
-    dt = float(np.float32(0.8 * acq.fd_cfl(8) * h / vmax))
+    dt = float(np.float32(0.8 * acq.fd_cfl(order) * h / vmax))
     r = p.inner()
     inner = np.zeros((p.n3, p.n2, p.n1), bool)
     inner[r[4]:r[5] + 1, r[2]:r[3] + 1, r[0]:r[1] + 1] = True
@@ -241,7 +241,7 @@ def adjoint_leg(args):
     src = np.array([acq.source_top_center(p, 1)], np.int64)
     rec = acq.receiver_line(p, 200, 1)
     wav = acq.ricker(nt, dt, f0)
-    log(f"adjoint: {n}^3 O8, nt {nt}, dt {dt:.3e} s, {len(rec)} receivers, checkpoints every {K} ({args.adj_format})")
+    log(f"adjoint: {n}^3 O{order}, nt {nt}, dt {dt:.3e} s, {len(rec)} receivers, checkpoints every {K} ({args.adj_format})")
 
     def dev(a):
         return capi.DeviceArray.from_numpy(np.ascontiguousarray(a))
@@ -274,7 +274,7 @@ def adjoint_leg(args):
             gr.free()
         return best, gout, J
 
-    t64, g64, J64 = grad_of(S64, d64, K, capi.HSX_CKPT_FP32)
+    t64, g64, J64 = grad_of(S64, d64, args.adj_fp64_every or K, capi.HSX_CKPT_FP32)
     ours_t, g_ours, J32 = grad_of(S32, dobs32, K, fmt, reps=3)
     full_t, g_full, _ = grad_of(S32, dobs32, 0, capi.HSX_CKPT_FP32, reps=2)
     nrm = np.linalg.norm(g64)
@@ -301,7 +301,7 @@ def adjoint_leg(args):
         torch.cuda.empty_cache()
         log(f"adjoint: plain PyTorch {plain_t:.3f} s (err {err_plain:.2e})")
     return {
-        "workload": f"one shot misfit gradient, {n}^3 FP32 FD order 8, {nt} steps, h 10 m, smooth velocity "
+        "workload": f"one shot misfit gradient, {n}^3 FP32 FD order {order}, {nt} steps, h 10 m, smooth velocity "
                     f"2000-4000 m/s, 8 Hz Ricker top centre, {len(rec)} receivers, data from a +-3% perturbed model",
         "s_per_shot": round(ours_t, 4),
         "checkpoints": f"every {K} steps, {args.adj_format} with per-snapshot max-abs scale, FP64 gradient",
@@ -314,6 +314,74 @@ def adjoint_leg(args):
         "full_storage_fp32_grad_rel_l2_vs_fp64": err_full,
         "fp64_reference_s": round(t64, 4),
         "misfit_fp32": J32, "misfit_fp64": J64,
+    }
+
+
+def batch_leg(args):
+    """Batched forward shots: args.batch_shots shots, cubes of N^3 inner points,
+    N drawn (seeded) in [192, 320] (multiples of 4), FP32 order 8, h = 10 m,
+    layered velocity 1500-4500 m/s with depth times (1 + 0.05 G), 10 Hz Ricker
+    one point below the top centre, receivers along n2 one point below the
+    top, nt steps at 0.8 x CFL.  All shots in one hsx_batch (one launch of each
+    kernel per step, the time loop replayed from a HIP graph), against the same
+    shots one after the other through hsx_forward."""
+    import numpy as np
+
+    from hpcscan_amd import acquisition as acq
+    from hpcscan_amd import capi
+    from hpcscan_amd.layout import make_layout
+
+    rng = np.random.default_rng(2024)
+    ns, nt, h, vmin, vmax, f0 = args.batch_shots, args.batch_nt, 10.0, 1500.0, 4500.0, 10.0
+    sizes = [int(4 * round(x / 4)) for x in rng.uniform(192, 320, ns)]
+    dt = float(np.float32(0.8 * acq.fd_cfl(8) * h / vmax))
+    wav = acq.ricker(nt, dt, f0).astype(np.float32).reshape(nt, 1)
+    grids, keep, shots, traces, pts = [], [], [], [], 0
+    t_setup = time.perf_counter()
+    for s, n in enumerate(sizes):
+        p = make_layout(3, (n, n, n), 8)
+        p.d = [h, h, h]
+        v = acq.layered_velocity(p, vmin, vmax, 0.05, seed=100 + s)
+        r = p.inner()
+        inner = np.zeros((p.n3, p.n2, p.n1), bool)
+        inner[r[4]:r[5] + 1, r[2]:r[3] + 1, r[0]:r[1] + 1] = True
+        coef = capi.DeviceArray.from_numpy(np.where(inner.ravel(), (v * dt) ** 2, 0.0).astype(np.float32))
+        src = capi.DeviceArray.from_numpy(np.array([acq.source_top_center(p, 1)], np.int64))
+        rec_h = acq.receiver_line(p, None, 1)
+        rec = capi.DeviceArray.from_numpy(rec_h)
+        w = capi.DeviceArray.from_numpy(wav)
+        tr = capi.DeviceArray(nt * len(rec_h) * 4)
+        shot = capi.make_shot(nt, dt, (src, 1), w, (rec, len(rec_h)))
+        grids.append(capi.hsx_grid(p))
+        keep.append((coef, src, rec, w, tr, len(rec_h)))
+        shots.append(shot)
+        traces.append(tr)
+        pts += n ** 3
+    log(f"batch: {ns} shots, N in [{min(sizes)}, {max(sizes)}], {pts / 1e9:.2f} G inner points, "
+        f"setup {time.perf_counter() - t_setup:.1f} s")
+    b = capi.Batch(grids, [k[0] for k in keep], shots, traces)
+    b.run(use_graph=True)                   # capture + first replay
+    t0 = time.perf_counter()
+    b.run(use_graph=True)
+    t_graph = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    b.run(use_graph=False)
+    t_nograph = time.perf_counter() - t0
+    batch_tr = [k[4].download(np.float32) for k in keep[:2]]
+    b.close()
+    # the same shots one by one
+    t0 = time.perf_counter()
+    for g, k, shot in zip(grids, keep, shots):
+        capi.forward(g, k[0], shot, k[4])
+    t_single = time.perf_counter() - t0
+    same = all(np.array_equal(batch_tr[i], keep[i][4].download(np.float32)) for i in range(len(batch_tr)))
+    log(f"batch: graph {t_graph:.3f} s, no graph {t_nograph:.3f} s, one by one {t_single:.3f} s, same={same}")
+    return {
+        "workload": f"{ns} forward shots, N^3 with N in [{min(sizes)}, {max(sizes)}], FP32 order 8, {nt} steps, "
+                    f"layered 1500-4500 m/s x (1 + 0.05 G), 10 Hz Ricker",
+        "gpoint_per_s": round(pts * nt / t_graph / 1e9, 3),
+        "s_graph": round(t_graph, 4), "s_no_graph": round(t_nograph, 4), "s_one_by_one": round(t_single, 4),
+        "traces_equal_single_shot": bool(same),
     }
 
 
@@ -339,10 +407,16 @@ def main():
                     help="load each plane's halo one plane ahead instead of with the queue front")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one full-grid copy kernel first (counter calibration for profiles)")
+    ap.add_argument("--batch", action="store_true", help="add the batched-shots leg (32 uneven shots)")
+    ap.add_argument("--batch-shots", type=int, default=32)
+    ap.add_argument("--batch-nt", type=int, default=1000)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient leg")
     ap.add_argument("--no-plain", action="store_true", help="adjoint leg without the plain PyTorch timing")
     ap.add_argument("--adj-n", type=int, default=200)
     ap.add_argument("--adj-nt", type=int, default=1000)
+    ap.add_argument("--adj-order", type=int, default=8, choices=[4, 8, 12, 16])
+    ap.add_argument("--adj-fp64-every", type=int, default=0,
+                    help="checkpoint interval of the FP64 reference gradient (0 = the same as --adj-every)")
     ap.add_argument("--adj-every", type=int, default=20)
     ap.add_argument("--adj-format", default="fp16", choices=["fp16", "bf16", "fp32"])
     args = ap.parse_args()
@@ -357,6 +431,7 @@ def main():
 
     gpu = gpu_leg(args)
     adj = None if args.no_adjoint else adjoint_leg(args)
+    bat = batch_leg(args) if args.batch else None
     cpu = None if args.no_cpu_baseline else cpu_leg(args)
     npt = gpu["npt"]
     pts_stencil = 6 * (args.order // 2) + 1
@@ -382,6 +457,7 @@ def main():
         "roofline": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in gpu["roofline"].items()},
         "cpu_baseline": cpu,
         "adjoint": adj,
+        "batch": bat,
         "extra": {"fused_kernel_ms": round(gpu["kernel_ms"], 4),
                   "hpcscan_apparent_GBps": round(gpu["value"] * (pts_stencil + 4) * 4, 1),
                   "device": gpu["device"]["name"], "arch": gpu["device"]["arch"]},

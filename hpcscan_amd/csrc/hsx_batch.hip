@@ -94,12 +94,13 @@ template <typename TS, int R>
 static int batch_setup_step(hsx_batch *b, const hsx_grid *grids, const void *const *coef)
 {
     constexpr int VW = 16 / sizeof(TS);
-    constexpr bool ring = Tiled<TS, TS, R, VW, 64, 16, MODE_AC2, true>::LDS <= TILE_LDS_MAX;
-    constexpr bool tall = Tiled<TS, TS, R, VW, 64, 16, MODE_AC2, false>::LDS <= TILE_LDS_MAX;
+    constexpr int LX = sizeof(TS) == 4 ? 64 : 32;   // FP64: 512-thread tiles (registers)
+    constexpr bool ring = Tiled<TS, TS, R, VW, LX, 16, MODE_AC2, true>::LDS <= TILE_LDS_MAX;
+    constexpr bool tall = Tiled<TS, TS, R, VW, LX, 16, MODE_AC2, false>::LDS <= TILE_LDS_MAX;
     constexpr int BY = (ring || tall) ? 16 : 8;
-    using K = Tiled<TS, TS, R, VW, 64, BY, MODE_AC2, ring>;
-    auto kern = k_tiled_batch<TS, TS, R, VW, 64, BY, MODE_AC2, ring>;
-    static const int per_cu = tiled_per_cu(kern, 64 * BY, K::LDS);
+    using K = Tiled<TS, TS, R, VW, LX, BY, MODE_AC2, ring>;
+    auto kern = k_tiled_batch<TS, TS, R, VW, LX, BY, MODE_AC2, ring>;
+    static const int per_cu = tiled_per_cu(kern, LX * BY, K::LDS);
     const int ns = b->nshots;
     const int64_t slots = std::max<int64_t>(1, (int64_t)per_cu * device_cus() / ns);
     std::vector<TiledArgs<TS>> args[2];
@@ -110,8 +111,8 @@ static int batch_setup_step(hsx_batch *b, const hsx_grid *grids, const void *con
     for (int s = 0; s < ns; s++) {
         void *A = F[s], *B = F[ns + s];
         // parity 0: B is the current wavefield, A the previous one (and the output)
-        int rc = fill_tiled_args<TS, TS, R, VW, 64, BY, MODE_AC2>(&grids[s], A, A, B, coef[s], slots, args[0][s]);
-        if (!rc) rc = fill_tiled_args<TS, TS, R, VW, 64, BY, MODE_AC2>(&grids[s], B, B, A, coef[s], slots, args[1][s]);
+        int rc = fill_tiled_args<TS, TS, R, VW, LX, BY, MODE_AC2>(&grids[s], A, A, B, coef[s], slots, args[0][s]);
+        if (!rc) rc = fill_tiled_args<TS, TS, R, VW, LX, BY, MODE_AC2>(&grids[s], B, B, A, coef[s], slots, args[1][s]);
         if (rc) return rc;
         off[s + 1] = off[s] + args[0][s].ntx * args[0][s].nty * args[0][s].nchunk;
     }
@@ -124,7 +125,7 @@ static int batch_setup_step(hsx_batch *b, const hsx_grid *grids, const void *con
     const int total = off[ns];
     const unsigned grid = (unsigned)(((int64_t)total + 7) / 8 * 8);
     b->step = [=](int parity, hipStream_t s) -> int {
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(64, BY), K::LDS, s, (const TiledArgs<TS> *)dargs[parity],
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(LX, BY), K::LDS, s, (const TiledArgs<TS> *)dargs[parity],
                            (const int *)doff, ns);
         HSX_LAUNCHED("k_tiled_batch");
         return HSX_OK;

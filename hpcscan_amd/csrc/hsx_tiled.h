@@ -217,8 +217,12 @@ __device__ __forceinline__ void tiled_body(const TiledArgs<TC> &a, int logical)
     const uint32_t hc_off = (uint32_t)((hc_i1 + i2 * n1) * ES);
 
     // ring slots owned by this thread
-    const int rc_idx = left ? tx : NHG + (tx - (LX - NHG));
-    auto ring_h_at = [&](int slot, int k) -> V * { return ring_h + ((slot * 2 * R + ty + k * BY) * LX + tx); };
+    // (threads that own no halo get in-range addresses too, so that the ring is
+    // read without branches; what they read is not used)
+    const int rc_idx = left ? tx : right ? NHG + (tx - (LX - NHG)) : 0;
+    auto ring_h_at = [&](int slot, int k) -> V * {
+        return ring_h + ((slot * 2 * R + (ty + k * BY) % (2 * R)) * LX + tx);
+    };
     auto ring_c_at = [&](int slot) -> V * { return ring_c + ((slot * BY + ty) * 2 * NHG + rc_idx); };
 
     auto load_halo = [&](int64_t z) {
@@ -281,31 +285,29 @@ __device__ __forceinline__ void tiled_body(const TiledArgs<TC> &a, int logical)
             if (K::NEED_Z) q[(s + 2 * R) % P] = front;
             const V cur = q[cz];
             const V prn_v = nprn, coef_v = ncoef;
-            V hrow_cur[NHR];
-            V hcol_cur;
-            if (RING) {
-                // halo of plane z from the ring, then the halo of plane z+R
-                // (registers) into the slot of plane z-1, free since then
-#pragma unroll
-                for (int k = 0; k < NHR; k++)
-                    if (hrow_ok[k]) hrow_cur[k] = *ring_h_at(rslot, k);
-                if (hc_ok) hcol_cur = *ring_c_at(rslot);
-                store_halo(rslot == 0 ? NR - 1 : rslot - 1);
-                rslot = rslot == NR - 1 ? 0 : rslot + 1;
-            } else {
-#pragma unroll
-                for (int k = 0; k < NHR; k++) hrow_cur[k] = hrow[k];
-                hcol_cur = hcol;
-            }
 
-            // stage plane z in LDS
+            // stage plane z in LDS: centre rows from the queue, halo from the
+            // ring (loaded R planes ago) or from the registers loaded last plane
             TS *pl = lds + buf * K::PLANE;
             if (K::NEED_LDS) {
                 *reinterpret_cast<V *>(pl + (ty + R) * W + HL + tx * VW) = cur;
+                if constexpr (RING) {
 #pragma unroll
-                for (int k = 0; k < NHR; k++)
-                    if (hrow_ok[k]) *reinterpret_cast<V *>(pl + hrow_lds[k] * W + HL + tx * VW) = hrow_cur[k];
-                if (hc_ok) *reinterpret_cast<V *>(pl + (ty + R) * W + hc_lds) = hcol_cur;
+                    for (int k = 0; k < NHR; k++) {
+                        const V t = *ring_h_at(rslot, k);
+                        if (hrow_ok[k]) *reinterpret_cast<V *>(pl + hrow_lds[k] * W + HL + tx * VW) = t;
+                    }
+                    const V t = *ring_c_at(rslot);
+                    if (hc_ok) *reinterpret_cast<V *>(pl + (ty + R) * W + hc_lds) = t;
+                    // the halo of plane z+R (registers) goes to the slot of plane z-1
+                    store_halo(rslot == 0 ? NR - 1 : rslot - 1);
+                    rslot = rslot == NR - 1 ? 0 : rslot + 1;
+                } else {
+#pragma unroll
+                    for (int k = 0; k < NHR; k++)
+                        if (hrow_ok[k]) *reinterpret_cast<V *>(pl + hrow_lds[k] * W + HL + tx * VW) = hrow[k];
+                    if (hc_ok) *reinterpret_cast<V *>(pl + (ty + R) * W + hc_lds) = hcol;
+                }
             }
 
             // prefetch plane z+1: queue front z+1+R, halos (of z+1+R with the
@@ -552,9 +554,12 @@ inline int launch_tiled(const hsx_grid *g, void *out, const void *prn_in, const 
 template <typename TS, typename TC, int R, int VW, int LX, int MODE>
 inline int launch_tiled_by(const hsx_grid *g, void *out, const void *prn_in, const void *u, const void *coef, hipStream_t s)
 {
-    constexpr bool ring_ok = LX >= 32 && Tiled<TS, TC, R, VW, LX, 16, MODE, true>::RING &&
+    // 1024-thread tiles leave 128 VGPRs per lane, too few for FP64 values (they spill)
+    constexpr bool fits_regs = (sizeof(TS) == 4 && sizeof(TC) == 4) || LX * 16 <= 512;
+    constexpr bool ring_ok = LX >= 32 && fits_regs && Tiled<TS, TC, R, VW, LX, 16, MODE, true>::RING &&
                              Tiled<TS, TC, R, VW, LX, 16, MODE, true>::LDS <= TILE_LDS_MAX;
-    constexpr bool tall_ok = LX >= 32 && Tiled<TS, TC, R, VW, LX, 16, MODE, false>::LDS <= TILE_LDS_MAX;
+    constexpr bool tall_ok = LX >= 32 && fits_regs &&
+                             Tiled<TS, TC, R, VW, LX, 16, MODE, false>::LDS <= TILE_LDS_MAX;
     if (g_tile.by != 8) {
         if constexpr (ring_ok) {
             if (!(g_tile.flags & HSX_TILE_NO_RING))

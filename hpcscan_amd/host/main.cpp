@@ -30,6 +30,10 @@ int main(int argc, char *argv[])
         if (t.run() == RTN_CODE_OK) numTestCase++;
     }
     {
+        TestCase_Grid t;
+        if (t.run() == RTN_CODE_OK) numTestCase++;
+    }
+    {
         TestCase_Propa t;
         if (t.run() == RTN_CODE_OK) numTestCase++;
     }

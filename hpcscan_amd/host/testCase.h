@@ -53,6 +53,13 @@ public:
     virtual Rtn_code run(void);
 };
 
+class TestCase_Grid : protected TestCase
+{
+public:
+    TestCase_Grid(void);
+    virtual Rtn_code run(void);
+};
+
 } // namespace hpcscan
 
 #endif

@@ -64,6 +64,19 @@ def test_same_report_as_cpu_reference(tmp_path, prec, args):
     assert ">> Test FAILED" not in gpu
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("dim", ["1", "2", "3"])
+def test_grid_test_case_on_gpu(tmp_path, prec, dim):
+    """-testCase Grid on the MI355X backend: the nine operations pass their
+    closed-form checks, as on the CPU oracle build"""
+    args = ["-testCase", "Grid", "-dim", dim, "-n1", "64", "-n2", "48", "-n3", "40", "-fdOrder", "8", "-ntry", "2"]
+    rc, gpu = run_bin(binary(prec), args + ["-testMode", "MI355X"], tmp_path)
+    rc2, cpu = run_bin(CPU_BIN[prec], args + ["-testMode", "CacheBlk"], tmp_path)
+    assert rc == 0 and rc2 == 0
+    assert gpu.count("Test PASSED") == 9 and "FAILED" not in gpu, gpu[-3000:]
+    assert cpu.count("Test PASSED") == 9
+
+
 def test_validation_matrix_zero_failed(tmp_path):
     """The stencil part of the reference's validation driver (script/testDriver.sh:79-141,
     233-321) on -testMode MI355X, one rank: 0 FAILED."""

@@ -52,6 +52,21 @@ def test_cpu_validation_subset_zero_failed(oracle, tmp_path, mode):
     assert failed == 0 and passed > 0
 
 
+@pytest.mark.parametrize("dim", ["1", "2", "3"])
+def test_cpu_grid_test_case(oracle, tmp_path, dim):
+    """-testCase Grid (src/testCase_Grid.cpp): nine operations, each checked
+    against its closed-form answer; 18 perf-log columns after the common ones."""
+    rc, out = run_bin(CPU, ["-testCase", "Grid", "-testMode", "CacheBlk", "-ntry", "2", "-dim", dim, "-n1", "37",
+                            "-n2", "29", "-n3", "21", "-fdOrder", "8"], tmp_path)
+    assert rc == 0
+    cases = re.findall(r"\* Case\s+(\S+)", out)
+    assert cases == ["Grid" + c for c in ("Fill", "MaxErr", "L1Err", "GetSumAbs", "GetSumAbsDiff", "GetMax",
+                                          "GetMin", "UpdatePressure", "ApplyBoundaryCondition")]
+    assert out.count("Test PASSED") == 9 and "FAILED" not in out
+    line = open(tmp_path / "hpcscan.perf.Grid.log").read().split()
+    assert len(line) == 4 + 9 + 18
+
+
 def test_split_and_standard_print_same_error(oracle, tmp_path):
     outs = []
     for prop in ("Ac2Standard", "Ac2SplitComp"):
