@@ -90,14 +90,10 @@ static int upload(hsx_batch *b, const void *host, size_t bytes, void **p)
 
 // the fused-step instance shared by every shot: 64 lanes, 16 rows with the
 // halo ring where it fits, else 16 rows without it, else 8 rows
-template <typename TS, int R>
-static int batch_setup_step(hsx_batch *b, const hsx_grid *grids, const void *const *coef)
+template <typename TS, int R, int LX, int BY, bool ring>
+static int batch_setup_shape(hsx_batch *b, const hsx_grid *grids, const void *const *coef)
 {
     constexpr int VW = 16 / sizeof(TS);
-    constexpr int LX = sizeof(TS) == 4 ? 64 : 32;   // FP64: 512-thread tiles (registers)
-    constexpr bool ring = Tiled<TS, TS, R, VW, LX, 16, MODE_AC2, true>::LDS <= TILE_LDS_MAX;
-    constexpr bool tall = Tiled<TS, TS, R, VW, LX, 16, MODE_AC2, false>::LDS <= TILE_LDS_MAX;
-    constexpr int BY = (ring || tall) ? 16 : 8;
     using K = Tiled<TS, TS, R, VW, LX, BY, MODE_AC2, ring>;
     auto kern = k_tiled_batch<TS, TS, R, VW, LX, BY, MODE_AC2, ring>;
     static const int per_cu = tiled_per_cu(kern, LX * BY, K::LDS);
@@ -131,6 +127,32 @@ static int batch_setup_step(hsx_batch *b, const hsx_grid *grids, const void *con
         return HSX_OK;
     };
     return HSX_OK;
+}
+
+// the fused-step instance shared by every shot: 16 rows with the halo ring
+// where its LDS fits and it keeps its values in registers, else 16 rows
+// without the ring, else 8 rows (64 lanes; 32 for FP64)
+template <typename TS, int R>
+static int batch_setup_step(hsx_batch *b, const hsx_grid *grids, const void *const *coef)
+{
+    constexpr int VW = 16 / sizeof(TS);
+    constexpr int LX = sizeof(TS) == 4 ? 64 : 32;
+    constexpr bool ring = Tiled<TS, TS, R, VW, LX, 16, MODE_AC2, true>::LDS <= TILE_LDS_MAX;
+    constexpr bool tall = Tiled<TS, TS, R, VW, LX, 16, MODE_AC2, false>::LDS <= TILE_LDS_MAX;
+    if constexpr (ring) {
+        if (!tiled_spills(k_tiled_batch<TS, TS, R, VW, LX, 16, MODE_AC2, true>))
+            return batch_setup_shape<TS, R, LX, 16, true>(b, grids, coef);
+    }
+    if constexpr (tall) {
+        if (!tiled_spills(k_tiled_batch<TS, TS, R, VW, LX, 16, MODE_AC2, false>))
+            return batch_setup_shape<TS, R, LX, 16, false>(b, grids, coef);
+    }
+    // 64 lanes x 16 rows spill (FP32, order 16): 32 lanes x 16 rows with the ring
+    if constexpr (LX == 64 && Tiled<TS, TS, R, VW, 32, 16, MODE_AC2, true>::LDS <= TILE_LDS_MAX) {
+        if (!tiled_spills(k_tiled_batch<TS, TS, R, VW, 32, 16, MODE_AC2, true>))
+            return batch_setup_shape<TS, R, 32, 16, true>(b, grids, coef);
+    }
+    return batch_setup_shape<TS, R, LX, 8, false>(b, grids, coef);
 }
 
 static int batch_record(hsx_batch *b, hipStream_t s)

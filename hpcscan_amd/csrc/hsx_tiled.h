@@ -495,8 +495,17 @@ inline int tiled_per_cu(KERN kern, int threads, size_t lds)
     return nb;
 }
 
+// whether a kernel instance keeps per-lane values in scratch memory (register spills)
+template <typename KERN>
+inline bool tiled_spills(KERN kern)
+{
+    hipFuncAttributes attr;
+    if (hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(kern)) != hipSuccess) return false;
+    return attr.localSizeBytes > 0;
+}
+
 // kernel arguments of one grid; slots = resident workgroups available to it
-template <typename TS, typename TC, int R, int VW, int LX, int BY, int MODE>
+template <typename TS, typename TC, int R, int VW, int LX, int BY, int MODE, bool RING = false>
 inline int fill_tiled_args(const hsx_grid *g, void *out, const void *prn_in, const void *u, const void *coef,
                            int64_t slots, TiledArgs<TC> &a)
 {
@@ -522,12 +531,12 @@ inline int fill_tiled_args(const hsx_grid *g, void *out, const void *prn_in, con
     const int64_t N3 = a.e3 - a.s3 + 1;
     const int64_t tiles = (int64_t)a.ntx * a.nty;
     const int64_t len = g_tile.chunk > 0 ? std::min<int64_t>(g_tile.chunk, N3)
-                        : BY == 16        ? chunk_one_round(N3, R, tiles, slots)
+                        : (BY == 16 || RING) ? chunk_one_round(N3, R, tiles, slots)
                                           : chunk_short(N3, R, tiles);
     a.chunk = (int)len;
     a.nchunk = (int)((N3 + len - 1) / len);
     const int streams = g_tile.flags & (HSX_TILE_CACHED | HSX_TILE_NT);
-    a.nt = streams == HSX_TILE_NT || (streams == 0 && BY == 16);
+    a.nt = streams == HSX_TILE_NT || (streams == 0 && (BY == 16 || RING));
     a.c = make_coefs<TC>(g);
     if (tiles * a.nchunk > (int64_t)1 << 30) return fail(HSX_ERR_UNSUPPORTED, "too many tiles");
     return HSX_OK;
@@ -541,7 +550,7 @@ inline int launch_tiled(const hsx_grid *g, void *out, const void *prn_in, const 
     auto kern = k_tiled<TS, TC, R, VW, LX, BY, MODE, RING>;
     static const int per_cu = tiled_per_cu(kern, LX * BY, K::LDS);
     TiledArgs<TC> a;
-    int rc = fill_tiled_args<TS, TC, R, VW, LX, BY, MODE>(g, out, prn_in, u, coef, (int64_t)per_cu * device_cus(), a);
+    int rc = fill_tiled_args<TS, TC, R, VW, LX, BY, MODE, RING>(g, out, prn_in, u, coef, (int64_t)per_cu * device_cus(), a);
     if (rc) return rc;
     const int64_t nblk = (int64_t)a.ntx * a.nty * a.nchunk;
     const unsigned grid = (unsigned)(((nblk + 7) / 8) * 8);
@@ -550,22 +559,56 @@ inline int launch_tiled(const hsx_grid *g, void *out, const void *prn_in, const 
     return HSX_OK;
 }
 
+// 16-row instance to use at LX lanes: 1 with the halo ring, 2 without, 0 none.
+// 1024-thread tiles leave 128 VGPRs per lane, too few for FP64 values; by
+// default an instance is used only if it keeps its values in registers (no
+// per-lane scratch); rows forced to 16 are used as asked
+template <typename TS, typename TC, int R, int VW, int LX, int MODE>
+struct TallFits {
+    static constexpr bool fits_regs = (sizeof(TS) == 4 && sizeof(TC) == 4) || LX * 16 <= 512;
+    static constexpr bool ring = LX >= 32 && fits_regs && Tiled<TS, TC, R, VW, LX, 16, MODE, true>::RING &&
+                                 Tiled<TS, TC, R, VW, LX, 16, MODE, true>::LDS <= TILE_LDS_MAX;
+    static constexpr bool tall = LX >= 32 && fits_regs &&
+                                 Tiled<TS, TC, R, VW, LX, 16, MODE, false>::LDS <= TILE_LDS_MAX;
+};
+
+template <typename TS, typename TC, int R, int VW, int LX, int MODE>
+inline int tall_variant()
+{
+    constexpr bool ring_ok = TallFits<TS, TC, R, VW, LX, MODE>::ring;
+    constexpr bool tall_ok = TallFits<TS, TC, R, VW, LX, MODE>::tall;
+    const bool forced = g_tile.by == 16;
+    if (g_tile.by == 8) return 0;
+    if constexpr (ring_ok) {
+        static const bool ring_spills = tiled_spills(k_tiled<TS, TC, R, VW, LX, 16, MODE, true>);
+        if (!(g_tile.flags & HSX_TILE_NO_RING) && (forced || !ring_spills)) return 1;
+    }
+    if constexpr (tall_ok) {
+        static const bool tall_spills = tiled_spills(k_tiled<TS, TC, R, VW, LX, 16, MODE, false>);
+        if (forced || !tall_spills) return 2;
+    }
+    return 0;
+}
+
 // BY = 16 (with the halo ring where it fits) for tiles of at least 32 lanes, else 8
 template <typename TS, typename TC, int R, int VW, int LX, int MODE>
 inline int launch_tiled_by(const hsx_grid *g, void *out, const void *prn_in, const void *u, const void *coef, hipStream_t s)
 {
-    // 1024-thread tiles leave 128 VGPRs per lane, too few for FP64 values (they spill)
-    constexpr bool fits_regs = (sizeof(TS) == 4 && sizeof(TC) == 4) || LX * 16 <= 512;
-    constexpr bool ring_ok = LX >= 32 && fits_regs && Tiled<TS, TC, R, VW, LX, 16, MODE, true>::RING &&
-                             Tiled<TS, TC, R, VW, LX, 16, MODE, true>::LDS <= TILE_LDS_MAX;
-    constexpr bool tall_ok = LX >= 32 && fits_regs &&
-                             Tiled<TS, TC, R, VW, LX, 16, MODE, false>::LDS <= TILE_LDS_MAX;
-    if (g_tile.by != 8) {
-        if constexpr (ring_ok) {
-            if (!(g_tile.flags & HSX_TILE_NO_RING))
-                return launch_tiled<TS, TC, R, VW, LX, 16, MODE, true>(g, out, prn_in, u, coef, s);
-        }
-        if constexpr (tall_ok) return launch_tiled<TS, TC, R, VW, LX, 16, MODE, false>(g, out, prn_in, u, coef, s);
+    const int tall = tall_variant<TS, TC, R, VW, LX, MODE>();
+    if constexpr (TallFits<TS, TC, R, VW, LX, MODE>::ring) {
+        if (tall == 1) return launch_tiled<TS, TC, R, VW, LX, 16, MODE, true>(g, out, prn_in, u, coef, s);
+    }
+    if constexpr (TallFits<TS, TC, R, VW, LX, MODE>::tall) {
+        if (tall == 2) return launch_tiled<TS, TC, R, VW, LX, 16, MODE, false>(g, out, prn_in, u, coef, s);
+    }
+    // 8 rows: the ring only when 8 rows are asked for (at 1000^3 FP32 order 8
+    // it measured 3.83 ms against 3.66 ms without it: its LDS halves the
+    // resident workgroups while the 8-row halo stays large)
+    constexpr bool ring8_ok = Tiled<TS, TC, R, VW, LX, 8, MODE, true>::RING &&
+                              Tiled<TS, TC, R, VW, LX, 8, MODE, true>::LDS <= TILE_LDS_MAX;
+    if constexpr (ring8_ok) {
+        if (g_tile.by == 8 && !(g_tile.flags & HSX_TILE_NO_RING))
+            return launch_tiled<TS, TC, R, VW, LX, 8, MODE, true>(g, out, prn_in, u, coef, s);
     }
     return launch_tiled<TS, TC, R, VW, LX, 8, MODE, false>(g, out, prn_in, u, coef, s);
 }
@@ -599,7 +642,10 @@ inline int run_tiled_shape(const hsx_grid *g, void *out, const void *prn_in, con
         return dispatch_R(R, [&](auto rc_) -> int {
             constexpr int RR = decltype(rc_)::value;
             if (lx == 16) return launch_tiled<TS, TC, RR, VW, 16, 8, MODE, false>(g, out, prn_in, u, coef, s);
-            if (lx == 32) return launch_tiled_by<TS, TC, RR, VW, 32, MODE>(g, out, prn_in, u, coef, s);
+            // 64 lanes whose 16-row instances all spill (order 16 FP32): 32 lanes x 16 rows
+            const bool narrow = lx == 64 && g_tile.lx == 0 && tall_variant<TS, TC, RR, VW, 64, MODE>() == 0 &&
+                                tall_variant<TS, TC, RR, VW, 32, MODE>() != 0;
+            if (lx == 32 || narrow) return launch_tiled_by<TS, TC, RR, VW, 32, MODE>(g, out, prn_in, u, coef, s);
             return launch_tiled_by<TS, TC, RR, VW, 64, MODE>(g, out, prn_in, u, coef, s);
         });
     });

@@ -106,8 +106,9 @@ extern "C" int hsx_autotune(const hsx_grid *g, int reps, int shape[4], float *be
     std::vector<TileCfg> cand = {
         {64, 16, 0, 0}, {64, 16, 0, HSX_TILE_NO_RING}, {64, 16, 0, HSX_TILE_CACHED},
         {32, 16, 0, 0}, {32, 16, 0, HSX_TILE_NO_RING},
-        {64, 8, 0, 0}, {64, 8, 0, HSX_TILE_NT}, {64, 8, (int)std::min<int64_t>(N3, 8 * R + 64), 0},
-        {32, 8, 0, 0}, {16, 8, 0, 0},
+        {64, 8, 0, 0}, {64, 8, 0, HSX_TILE_NO_RING}, {64, 8, 0, HSX_TILE_NO_RING | HSX_TILE_NT},
+        {64, 8, (int)std::min<int64_t>(N3, 8 * R + 64), HSX_TILE_NO_RING},
+        {32, 8, 0, 0}, {32, 8, 0, HSX_TILE_NO_RING}, {16, 8, 0, 0},
     };
     const TileCfg saved = g_tile;
     TileCfg best = {0, 0, 0, 0};

@@ -312,7 +312,11 @@ TILED_CASES = [
     (8, "f32", 130, 17, 40, 32, 13, 8, 1),
     (8, "f32", 130, 37, 40, 32, 13, 16, 2),   # 16 rows with the halo ring, partial last tile row
     (8, "f32", 130, 37, 40, 32, 13, 16, 6),   # 16 rows without the ring
-    (8, "f32", 200, 24, 31, 64, 0, 8, 0),     # one wide tile, default chunk
+    (8, "f32", 200, 24, 31, 64, 0, 8, 0),     # one wide tile, default chunk, 8 rows with the ring
+    (8, "f32", 200, 37, 31, 64, 0, 8, 4),     # 8 rows without the ring
+    (8, "f32", 200, 37, 31, 64, 11, 8, 1),    # 8 rows, ring, cached streams, chunk not a multiple of 2R+1
+    (2, "f32", 140, 13, 22, 64, 5, 8, 0),     # R = 1 ring at 8 rows
+    (16, "f32", 140, 19, 20, 32, 0, 8, 0),    # R = 8 ring at 8 rows: two n2 halo rows per thread
     (8, "f32", 200, 40, 31, 64, 7, 16, 1),
     (8, "f32", 200, 40, 3, 64, 0, 16, 0),     # fewer planes than R: ring filled past the chunk
     (2, "f32", 69, 20, 22, 16, 7, 0, 0),      # odd halo: n1 offset, tile origin below the inner start
@@ -326,6 +330,8 @@ TILED_CASES = [
     (4, "f64", 70, 21, 27, 16, 5, 0, 0),      # VW = 2
     (4, "f64", 150, 37, 27, 64, 9, 16, 2),
     (4, "f64", 150, 37, 27, 64, 9, 16, 4),
+    (4, "f64", 150, 19, 27, 64, 0, 8, 0),     # FP64 ring at 8 rows
+    (10, "f64", 150, 23, 25, 32, 7, 8, 2),
     (12, "f64", 66, 30, 26, 32, 11, 0, 1),
     (8, "f32c64", 72, 18, 21, 16, 5, 0, 0),   # FP32 storage, FP64 compute
     (8, "f32c64", 140, 35, 21, 64, 5, 16, 2),
