@@ -3,8 +3,10 @@
 
 One step is one time step of hpcscan's propagator (src/propagator_Ac2.cpp:366-393):
 free-surface boundary on prc, then the fused Laplacian + leapfrog on prn, over
-the 1000^3 inner grid (eigen mode 2,2,2 initial state, dt = CFL limit).  Grids
-are resident in HBM before timing.  Prints one JSON line.
+the 1000^3 inner grid (eigen mode 2,2,2 initial state, dt = CFL limit); the
+free surface runs as two launches of its own before each step (--fold: folded
+into the fused kernel instead, slower on this device).  Grids are resident in
+HBM before timing.  Prints one JSON line.
 
   value      GPoint/s = inner points x steps / wall time of the timed steps
   roofline   the fused step kernel: 16 algorithmic bytes per inner point
@@ -99,13 +101,31 @@ def gpu_leg(args):
     capi.fill_separable(g, prc, r_all, *tabs, math.sin(PI_REF * float(np.float32(-dt)) * gamma))
     capi.sync()
 
+    # a step is the reference's free surface on prc then the fused update of prn;
+    # folded (--fold): the first step applies the free surface to the initial
+    # prc and writes the new field's halos, every later step reads the owed
+    # boundary-plane zeros of prc / prn inside the fused kernel
+    nstep = [0]
+    folded = args.fold
+
     def step(a, b, ev=None):
-        capi.free_surface(g, b)
-        if ev:
-            ev[0].record()
-        capi.ac2_step(g, a, b, coef)
+        if folded:
+            fold = capi.HSX_FOLD_HALO
+            if nstep[0] == 0:
+                capi.free_surface(g, b)
+            else:
+                fold |= capi.HSX_FOLD_PRC | capi.HSX_FOLD_PRN
+            if ev:
+                ev[0].record()
+            capi.ac2_step_fs(g, a, b, coef, fold)
+        else:
+            capi.free_surface(g, b)
+            if ev:
+                ev[0].record()
+            capi.ac2_step(g, a, b, coef)
         if ev:
             ev[1].record()
+        nstep[0] += 1
 
     if args.calibrate_copy:
         # one plain copy of a whole grid (npoint*4 bytes read + written): the
@@ -147,7 +167,9 @@ def gpu_leg(args):
         (0, 0, 0, "default", False)
     if os.path.exists(TRAFFIC_FILE) and (n, order, args.kernel) == (1000, 8, "auto") and default_shape:
         try:
-            traffic = json.load(open(TRAFFIC_FILE)).get("hbm_bytes_per_launch")
+            tf = json.load(open(TRAFFIC_FILE))
+            if bool(tf.get("free_surface_folded", False)) == args.fold:
+                traffic = tf.get("hbm_bytes_per_launch")
         except Exception:
             traffic = None
     return {
@@ -405,6 +427,8 @@ def main():
                     help="n3 planes streamed per workgroup (0 = library choice)")
     ap.add_argument("--streams", default="default", choices=["default", "cached", "nt"],
                     help="cached or nontemporal prn/coef/output accesses of the fused kernel")
+    ap.add_argument("--fold", action="store_true",
+                    help="free surface folded into the fused step (hsx_ac2_step_fs) instead of its own two launches")
     ap.add_argument("--no-ring", action="store_true",
                     help="load each plane's halo one plane ahead instead of with the queue front")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -455,7 +479,8 @@ def main():
                    "grid": [args.n] * 3, "fd_order": args.order, "propagator": "Ac2Standard",
                    "inner_points": npt, "kernel": args.kernel,
                    "tile_lx": args.tile_lx, "tile_rows": args.tile_rows, "tile_chunk": args.tile_chunk,
-                   "streams": args.streams, "halo_ring": not args.no_ring},
+                   "streams": args.streams, "halo_ring": not args.no_ring,
+                   "free_surface": "folded into the step" if args.fold else "two launches before the step"},
         "roofline": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in gpu["roofline"].items()},
         "cpu_baseline": cpu,
         "adjoint": adj,

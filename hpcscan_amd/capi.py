@@ -21,6 +21,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libhpcscan_mi355x.so")
 
 HSX_OK = 0
+HSX_ERR_INVALID, HSX_ERR_HIP, HSX_ERR_NOMEM, HSX_ERR_UNSUPPORTED = -1, -2, -3, -4
 HSX_FP32, HSX_FP64 = 0, 1
 HSX_KERNEL_AUTO, HSX_KERNEL_REF, HSX_KERNEL_TILED = 0, 1, 2
 HSX_TILE_CACHED, HSX_TILE_NT, HSX_TILE_NO_RING = 1, 2, 4
@@ -94,6 +95,7 @@ SIGNATURES = {
     "hsx_max_err": (_i, [_G, _vp, _vp, _R, _d, _P(_d), _vp]),
     "hsx_min_max": (_i, [_G, _vp, _R, _P(_d), _vp]),
     "hsx_ac2_step_to": (_i, [_G, _vp, _vp, _vp, _vp, _vp]),
+    "hsx_ac2_step_fs": (_i, [_G, _vp, _vp, _vp, _i, _i, _vp]),
     "hsx_inject": (_i, [_G, _vp, _i64, _vp, _vp, _vp, _d, _i, _vp]),
     "hsx_gather": (_i, [_G, _vp, _i64, _vp, _vp, _vp]),
     "hsx_imaging": (_i, [_G, _vp, _vp, _vp, _vp, _vp, _R, _d, _i64, _d, _vp]),
@@ -121,6 +123,7 @@ class HsxAdjointOpts(ctypes.Structure):
 
 
 HSX_INJECT_ADD, HSX_INJECT_SET = 0, 1
+HSX_FOLD_PRC, HSX_FOLD_PRN, HSX_FOLD_HALO = 1, 2, 4
 HSX_CKPT_FP32, HSX_CKPT_FP16, HSX_CKPT_BF16 = 0, 1, 2
 
 _lib = None
@@ -332,6 +335,12 @@ def sync(stream=None):
 # ---- modeling / adjoint ------------------------------------------------------------
 def ac2_step_to(g: HsxGrid, out, prn, prc, coef, stream=None):
     check(lib().hsx_ac2_step_to(ctypes.byref(g), dptr(out), dptr(prn), dptr(prc), dptr(coef), stream))
+
+
+def ac2_step_fs(g: HsxGrid, prn, prc, coef, fold, face_mask=ALL_FACES, stream=None):
+    """Free surface folded into the step (hsx_ac2_step_fs). Raises HsxError with
+    rc HSX_ERR_UNSUPPORTED when the grid does not take the folded kernel."""
+    check(lib().hsx_ac2_step_fs(ctypes.byref(g), dptr(prn), dptr(prc), dptr(coef), face_mask, fold, stream))
 
 
 def inject(g: HsxGrid, u, n, idx, val, w=None, scale=1.0, mode=HSX_INJECT_ADD, stream=None):

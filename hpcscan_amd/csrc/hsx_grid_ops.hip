@@ -265,11 +265,12 @@ int make_faceset(const hsx_grid *g, int face_mask, FaceSet *fs, bool *closed_for
             *most_plane = std::max(*most_plane, plane);
         }
         // closed form: the mirrors of each face stay strictly between the two
-        // boundary planes of the axis (both computed as the sequential phases do)
+        // boundary planes of the axis (both computed as the sequential phases do);
+        // an axis with no face in the mask adds no phase and no condition
         const int64_t z0 = g->halo1[2 * axis + 1] + 1, z1 = g->halo2[2 * axis] - 1;
         const int64_t h1 = g->halo1[2 * axis + 1] - g->halo1[2 * axis] + 1;
         const int64_t h2 = g->halo2[2 * axis + 1] - g->halo2[2 * axis] + 1;
-        if (z1 - z0 <= (h1 > h2 ? h1 : h2)) *closed_form = false;
+        if (((fs->zmask >> (2 * axis)) & 3) && z1 - z0 <= (h1 > h2 ? h1 : h2)) *closed_form = false;
     }
     return HSX_OK;
 }

@@ -133,9 +133,13 @@ inline int dispatch_dim(int dim, F &&f)
     }
 }
 
-// tiled kernels (hsx_stencil_tiled.hip); return HSX_ERR_UNSUPPORTED when the
+// tiled kernels (hsx_tiled_*.hip); return HSX_ERR_UNSUPPORTED when the
 // grid does not fit their assumptions so the caller uses the reference kernels
 int tiled_ac2_step(const hsx_grid *g, void *out, const void *prn, const void *prc, const void *coef, hipStream_t s);
+// in-place step with the free surface folded in (flags: FOLD_* of hsx_tiled.h;
+// fb: boundary plane of each folded face, -1 for the others)
+int tiled_ac2_step_fold(const hsx_grid *g, void *prn, const void *prc, const void *coef, int flags,
+                        const int64_t fb[6], hipStream_t s);
 int tiled_laplacian(const hsx_grid *g, const void *u, void *w, hipStream_t s);
 int tiled_fd_d2_x(const hsx_grid *g, const void *u, void *w, hipStream_t s);
 int tiled_fd_d2_y(const hsx_grid *g, const void *u, void *w, hipStream_t s);

@@ -169,4 +169,24 @@ int hsx_ac2_step_to(const hsx_grid *g, void *out, const void *prn, const void *p
     });
 }
 
+int hsx_ac2_step_fs(const hsx_grid *g, void *prn, const void *prc, const void *coef, int face_mask, int fold,
+                    void *stream)
+{
+    if (fold & ~(HSX_FOLD_PRC | HSX_FOLD_PRN | HSX_FOLD_HALO)) return fail(HSX_ERR_INVALID, "fold flags 0x%x", fold);
+    if (fold == 0) return hsx_ac2_step(g, prn, prc, coef, stream);
+    int rc = stencil_checks(g, {{prn, "prn"}, {prc, "prc"}, {coef, "coef"}});
+    if (rc) return rc;
+    if (prn == prc) return fail(HSX_ERR_INVALID, "prn and prc must be different grids");
+    if (g->dim != 3 || kernel_family() == HSX_KERNEL_REF)
+        return fail(HSX_ERR_UNSUPPORTED, "folded free surface: 3D grids on the tiled kernels only");
+    FaceSet fs;
+    bool closed = false;
+    int64_t most = 0, most_plane = 0;
+    if ((rc = make_faceset(g, face_mask, &fs, &closed, &most, &most_plane))) return rc;
+    if (!closed) return fail(HSX_ERR_UNSUPPORTED, "folded free surface: fewer than R+2 inner points on an axis");
+    int64_t fb[6];
+    for (int k = 0; k < 6; k++) fb[k] = (fs.zmask >> k) & 1 ? fs.zidx[k / 2][k % 2] : -1;
+    return tiled_ac2_step_fold(g, prn, prc, coef, fold, fb, stream_of(stream));
+}
+
 } // extern "C"

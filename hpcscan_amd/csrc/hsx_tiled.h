@@ -46,8 +46,16 @@ struct TiledArgs {
     int64_t t1s;              // n1 origin of the first tile (multiple of VW)
     int ntx, nty, nchunk, chunk;
     int nt;                   // 1: prn / coef / output streamed with nontemporal accesses
+    // MODE_AC2: the free surface folded into the step (0 = none):
+    //  FOLD_PRC  prc is read as if its boundary planes were zero,
+    //  FOLD_PRN  prn likewise,
+    //  FOLD_HALO the output's halos of the folded faces are written (-mirror).
+    int fold;
+    int fb[6];                // boundary plane of each folded face (axis-major, low side first), -1 if not folded
     Coefs<TC> c;
 };
+
+constexpr int FOLD_PRC = HSX_FOLD_PRC, FOLD_PRN = HSX_FOLD_PRN, FOLD_HALO = HSX_FOLD_HALO;
 
 constexpr int ceil_div(int a, int b) { return (a + b - 1) / b; }
 constexpr int round_up(int a, int b) { return ceil_div(a, b) * b; }
@@ -138,7 +146,7 @@ __device__ __forceinline__ int xcd_logical(int total)
 }
 
 // one workgroup: tile / chunk number 'logical' of the grid described by a
-template <typename TS, typename TC, int R, int VW, int LX, int BY, int MODE, bool RING_>
+template <typename TS, typename TC, int R, int VW, int LX, int BY, int MODE, bool RING_, bool FOLD = false>
 __device__ __forceinline__ void tiled_body(const TiledArgs<TC> &a, int logical)
 {
     using K = Tiled<TS, TC, R, VW, LX, BY, MODE, RING_>;
@@ -179,6 +187,53 @@ __device__ __forceinline__ void tiled_body(const TiledArgs<TC> &a, int logical)
     };
     const uint32_t col = (uint32_t)((i1 + i2 * n1) * ES);
 
+    // ---- folded free surface (FOLD instances; the closed form of
+    // Grid::applyBoundaryCondition ANTI_MIRROR, src/grid.cpp:3065-3230): an inner
+    // point on a boundary plane of a folded face reads as +0; halos are read as
+    // stored (the previous folded step wrote them)
+    constexpr bool FK = FOLD && MODE == MODE_AC2;
+    const int fold = FK ? a.fold : 0;
+    const int64_t tw = (int64_t)LX * VW;
+    auto in_range = [](int64_t x, int64_t lo, int64_t hi) { return x >= lo && x < hi; };
+    // the tile (with its halo) holds a boundary column / row of a folded face (uniform)
+    const bool edge_t = FK && (in_range(a.fb[0], t1 - HL, t1 + tw + HL) || in_range(a.fb[1], t1 - HL, t1 + tw + HL) ||
+                               in_range(a.fb[2], t2s - R, t2s + BY + R) || in_range(a.fb[3], t2s - R, t2s + BY + R));
+    auto bplane = [&](int64_t z) { return (int)z == a.fb[4] || (int)z == a.fb[5]; };
+    // per-lane flags (VW <= 4 elements):
+    //   bits 0-3 / 4-7     element column inner / on a folded boundary column
+    //   bits 8 / 9         row inner / on a folded boundary row
+    //   bits 16-19 / 20-23 the same as 0-7 for the n1 halo group of this lane
+    //   bits 24+2k / 25+2k n2 halo row k inner / on a folded boundary row
+    uint32_t fl = 0;
+    if constexpr (FK) {
+#pragma unroll
+        for (int v = 0; v < VW; v++) {
+            const int64_t c = i1 + v;
+            if (c >= a.s1 && c <= a.e1) fl |= 1u << v;
+            if (c == a.fb[0] || c == a.fb[1]) fl |= 1u << (4 + v);
+        }
+        if (i2 >= a.s2 && i2 <= a.e2) fl |= 1u << 8;
+        if (i2 == a.fb[2] || i2 == a.fb[3]) fl |= 1u << 9;
+    }
+    // element v (column bits cb: inner | boundary << 4; row bits rb: inner |
+    // boundary << 1) of plane z becomes +0 when the point is inner and lies on a
+    // folded boundary plane; the uniform test skips most tiles and planes
+    // The flags are made opaque to the compiler at each use (an empty asm
+    // statement), so that what is derived from them is recomputed in place
+    // instead of being hoisted out of the plane loop and kept live; the zeroing
+    // is a bit mask on the value (VALU only, no lane masks).
+    using UB = std::conditional_t<sizeof(TS) == 8, uint64_t, uint32_t>;
+    auto zero_fold = [&](V &x, uint32_t cb, uint32_t rb, int64_t z) {
+        if (!(z >= a.s3 && z <= a.e3 && (edge_t || bplane(z)))) return;
+        const uint32_t pz = bplane(z) ? 1u : 0u;
+        asm volatile("" : "+v"(cb), "+v"(rb));
+#pragma unroll
+        for (int v = 0; v < VW; v++) {
+            const uint32_t zb = (cb >> v) & rb & ((cb >> (4 + v)) | (rb >> 1) | pz) & 1u;
+            x.v[v] = __builtin_bit_cast(TS, __builtin_bit_cast(UB, x.v[v]) & ((UB)zb - 1));
+        }
+    };
+
     // ---- queue: slot (z - z0 + R) % P holds plane z
     V q[P];
 #pragma unroll
@@ -186,6 +241,10 @@ __device__ __forceinline__ void tiled_body(const TiledArgs<TC> &a, int logical)
 #pragma unroll
     for (int k = 0; k < 2 * R; k++)        // planes z0-R .. z0+R-1
         if (col_ok) q[k] = vload<TS, VW>(at(ub, z0 - R + k, col));
+    if (fold & FOLD_PRC) {
+#pragma unroll
+        for (int k = 0; k < 2 * R; k++) zero_fold(q[k], fl & 0xff, (fl >> 8) & 3, z0 - R + k);
+    }
 
     // prefetch registers for the first plane
     V front = vzero<TS, VW>();
@@ -231,6 +290,27 @@ __device__ __forceinline__ void tiled_body(const TiledArgs<TC> &a, int logical)
             if (hrow_ok[k]) hrow[k] = vload<TS, VW>(at(ub, z, hrow_off[k]));
         if (hc_ok) hcol = vload<TS, VW>(at(ub, z, hc_off));
     };
+    // the free-surface zeros in the halo just loaded for plane z
+    if constexpr (FK) {
+#pragma unroll
+        for (int k = 0; k < NHR; k++) {
+            const int64_t hi2 = t2s + hrow_lds[k] - R;
+            if (hi2 >= a.s2 && hi2 <= a.e2) fl |= 1u << (24 + 2 * k);
+            if (hi2 == a.fb[2] || hi2 == a.fb[3]) fl |= 1u << (25 + 2 * k);
+        }
+#pragma unroll
+        for (int v = 0; v < VW; v++) {
+            const int64_t c = hc_i1 + v;
+            if (c >= a.s1 && c <= a.e1) fl |= 1u << (16 + v);
+            if (c == a.fb[0] || c == a.fb[1]) fl |= 1u << (20 + v);
+        }
+    }
+    auto fold_halo = [&](int64_t z) {
+        if (!(fold & FOLD_PRC)) return;
+#pragma unroll
+        for (int k = 0; k < NHR; k++) zero_fold(hrow[k], fl & 0xff, (fl >> (24 + 2 * k)) & 3, z);
+        zero_fold(hcol, (fl >> 16) & 0xff, (fl >> 8) & 3, z);
+    };
     auto store_halo = [&](int slot) {
 #pragma unroll
         for (int k = 0; k < NHR; k++)
@@ -252,6 +332,7 @@ __device__ __forceinline__ void tiled_body(const TiledArgs<TC> &a, int logical)
         // halos of planes z0 .. z0+R-1 into the ring, plane z0+R in registers
         for (int j = 0; j < R; j++) {
             load_halo(z0 + j);
+            fold_halo(z0 + j);
             store_halo((int)((z0 + j) % NR));
         }
         load_halo(z0 + R);
@@ -271,6 +352,60 @@ __device__ __forceinline__ void tiled_body(const TiledArgs<TC> &a, int logical)
         all_inner &= inner[v];
     }
 
+    // FOLD_HALO: each output point within R of a folded face is also written,
+    // negated, to its mirror in the halo of that face; -0 instead where the
+    // point lies on a boundary plane of an earlier axis (the closed form of the
+    // reference's six face phases)
+    auto near = [&](int64_t b, int64_t lo, int64_t hi, int side) {   // [lo, hi) meets the R points inside face b
+        return b >= 0 && (side == 0 ? lo <= b + R && hi > b + 1 : lo <= b - 1 && hi > b - R);
+    };
+    const bool near1 = FK && (near(a.fb[0], t1, t1 + tw, 0) || near(a.fb[1], t1, t1 + tw, 1));
+    const bool near2 = FK && (near(a.fb[2], t2s, t2s + BY, 0) || near(a.fb[3], t2s, t2s + BY, 1));
+    // (flags and thread indices pass through empty asm statements: what is
+    // derived from them is computed here, when a face is near, instead of
+    // being hoisted out of the plane loop into registers)
+    auto mirror_out = [&](const V &res, int64_t z) {
+        uint32_t f = fl;
+        asm volatile("" : "+v"(f));
+        for (int side = 0; side < 2; side++) {
+            const int64_t b = a.fb[4 + side];
+            const int64_t d = side == 0 ? z - b : b - z;
+            if (b < 0 || d < 1 || d > R) continue;                        // uniform
+            TS *o = reinterpret_cast<TS *>((ob + (2 * b - z) * pstride) + (uint64_t)col);
+#pragma unroll
+            for (int v = 0; v < VW; v++)
+                if ((f >> v) & (f >> 8) & 1) o[v] = (((f >> (4 + v)) | (f >> 9)) & 1) ? -(TS)0 : -res.v[v];
+        }
+        if (near2) {
+            uint32_t y = threadIdx.y;
+            asm volatile("" : "+v"(y));
+            const int r = (int)t2s + (int)y;
+            for (int side = 0; side < 2; side++) {
+                const int b = a.fb[2 + side];
+                const int d = side == 0 ? r - b : b - r;
+                if (b < 0 || d < 1 || d > R) continue;
+                TS *o = reinterpret_cast<TS *>((ob + z * pstride) + (uint64_t)col +
+                                               (int64_t)(side == 0 ? -2 * d : 2 * d) * n1 * ES);
+#pragma unroll
+                for (int v = 0; v < VW; v++)
+                    if ((f >> v) & (f >> 8) & 1) o[v] = ((f >> (4 + v)) & 1) ? -(TS)0 : -res.v[v];
+            }
+        }
+        if (near1) {
+            uint32_t x = threadIdx.x;
+            asm volatile("" : "+v"(x));
+            const int c0 = (int)t1 + (int)x * VW;
+            TS *o = reinterpret_cast<TS *>((ob + z * pstride) + (uint64_t)col);
+#pragma unroll
+            for (int v = 0; v < VW; v++) {
+                if (!((f >> v) & (f >> 8) & 1)) continue;
+                const int dl = c0 + v - a.fb[0], dh = a.fb[1] - (c0 + v);
+                if (a.fb[0] >= 0 && dl >= 1 && dl <= R) o[v - 2 * dl] = -res.v[v];
+                if (a.fb[1] >= 0 && dh >= 1 && dh <= R) o[v + 2 * dh] = -res.v[v];
+            }
+        }
+    };
+
     const TC *A = a.c.A;
     int buf = 0;
     int rslot = (int)(z0 % NR);              // ring slot of plane z
@@ -282,9 +417,14 @@ __device__ __forceinline__ void tiled_body(const TiledArgs<TC> &a, int logical)
             if (z >= z1) return;                 // uniform over the workgroup
             constexpr int cz = (s + R) % P;      // slot of plane z
             // the front loaded during the previous plane is plane z+R
-            if (K::NEED_Z) q[(s + 2 * R) % P] = front;
+            if (K::NEED_Z) {
+                q[(s + 2 * R) % P] = front;
+                if (fold & FOLD_PRC) zero_fold(q[(s + 2 * R) % P], fl & 0xff, (fl >> 8) & 3, z + R);
+            }
             const V cur = q[cz];
-            const V prn_v = nprn, coef_v = ncoef;
+            V prn_v = nprn;
+            const V coef_v = ncoef;
+            if (fold & FOLD_PRN) zero_fold(prn_v, fl & 0xff, (fl >> 8) & 3, z);
 
             // stage plane z in LDS: centre rows from the queue, halo from the
             // ring (loaded R planes ago) or from the registers loaded last plane
@@ -300,9 +440,11 @@ __device__ __forceinline__ void tiled_body(const TiledArgs<TC> &a, int logical)
                     const V t = *ring_c_at(rslot);
                     if (hc_ok) *reinterpret_cast<V *>(pl + (ty + R) * W + hc_lds) = t;
                     // the halo of plane z+R (registers) goes to the slot of plane z-1
+                    fold_halo(z + R);
                     store_halo(rslot == 0 ? NR - 1 : rslot - 1);
                     rslot = rslot == NR - 1 ? 0 : rslot + 1;
                 } else {
+                    fold_halo(z);
 #pragma unroll
                     for (int k = 0; k < NHR; k++)
                         if (hrow_ok[k]) *reinterpret_cast<V *>(pl + hrow_lds[k] * W + HL + tx * VW) = hrow[k];
@@ -324,7 +466,10 @@ __device__ __forceinline__ void tiled_body(const TiledArgs<TC> &a, int logical)
             if (any_inner) {
                 // n1 neighbours: the row segment [tx*VW, tx*VW + VW + 2 HL) of the LDS row
                 TS xr[2 * HL + VW];
-                TS yu[R][VW], yd[R][VW];
+                // n2: accumulated row pair by row pair for all VW points at once
+                // (the order of d2_values for each point), so that only one pair
+                // of LDS rows is live at a time
+                TC ay[VW];
                 if (K::NEED_X) {
                     const TS *rowp = pl + (ty + R) * W + tx * VW;
 #pragma unroll
@@ -336,13 +481,20 @@ __device__ __forceinline__ void tiled_body(const TiledArgs<TC> &a, int logical)
                 }
                 if (K::NEED_Y) {
 #pragma unroll
+                    for (int v = 0; v < VW; v++) ay[v] = A[0] * (TC)cur.v[v];
+#pragma unroll
                     for (int j = 1; j <= R; j++) {
                         const V tu = *reinterpret_cast<const V *>(pl + (ty + R + j) * W + HL + tx * VW);
                         const V td = *reinterpret_cast<const V *>(pl + (ty + R - j) * W + HL + tx * VW);
 #pragma unroll
                         for (int v = 0; v < VW; v++) {
-                            yu[j - 1][v] = tu.v[v];
-                            yd[j - 1][v] = td.v[v];
+                            if (R == 1) {
+                                ay[v] = ay[v] + (TC)tu.v[v];
+                                ay[v] = ay[v] + (TC)td.v[v];
+                            } else {
+                                const TS pair = tu.v[v] + td.v[v];
+                                ay[v] = ay[v] + A[j] * (TC)pair;
+                            }
                         }
                     }
                 }
@@ -360,14 +512,7 @@ __device__ __forceinline__ void tiled_body(const TiledArgs<TC> &a, int logical)
                         }
                         d1 = d2_values<TS, TC, R>(c0, p, m, A, a.c.inv2[0]);
                     }
-                    if (K::NEED_Y) {
-#pragma unroll
-                        for (int j = 1; j <= R; j++) {
-                            p[j - 1] = yu[j - 1][v];
-                            m[j - 1] = yd[j - 1][v];
-                        }
-                        d2 = d2_values<TS, TC, R>(c0, p, m, A, a.c.inv2[1]);
-                    }
+                    if (K::NEED_Y) d2 = ay[v] * a.c.inv2[1];
                     if (K::NEED_Z) {
 #pragma unroll
                         for (int j = 1; j <= R; j++) {
@@ -402,19 +547,22 @@ __device__ __forceinline__ void tiled_body(const TiledArgs<TC> &a, int logical)
                     for (int v = 0; v < VW; v++)
                         if (inner[v]) outp[v] = res.v[v];
                 }
+                if constexpr (FK) {
+                    if (fold & FOLD_HALO) mirror_out(res, z);
+                }
             }
             buf ^= 1;
         });
     }
 }
 
-template <typename TS, typename TC, int R, int VW, int LX, int BY, int MODE, bool RING_>
+template <typename TS, typename TC, int R, int VW, int LX, int BY, int MODE, bool RING_, bool FOLD = false>
 __global__ void __launch_bounds__(LX *BY) k_tiled(TiledArgs<TC> a)
 {
     const int nblk = a.ntx * a.nty * a.nchunk;
     const int logical = xcd_logical(nblk);
     if (logical >= nblk) return;                       // whole block leaves, before any barrier
-    tiled_body<TS, TC, R, VW, LX, BY, MODE, RING_>(a, logical);
+    tiled_body<TS, TC, R, VW, LX, BY, MODE, RING_, FOLD>(a, logical);
 }
 
 // Many grids in one launch (the batched shots): off[s] = first logical block
@@ -447,6 +595,13 @@ struct TileCfg {
     int lx, by, chunk, flags;
 };
 extern TileCfg g_tile;                // defined in hsx_tiled_ac2.hip
+
+// free surface folded into the next MODE_AC2 launch of this thread (hsx_ac2_step_fs)
+struct FoldReq {
+    int flags;           // FOLD_* bits, 0 = none
+    int64_t fb[6];
+};
+extern thread_local FoldReq t_fold;   // defined in hsx_tiled_ac2.hip
 constexpr int TILE_LDS_MAX = 163840;  // gfx950: 160 KiB of LDS per workgroup
 
 inline int pick_lx(const hsx_grid *g, int vw)
@@ -537,17 +692,19 @@ inline int fill_tiled_args(const hsx_grid *g, void *out, const void *prn_in, con
     a.nchunk = (int)((N3 + len - 1) / len);
     const int streams = g_tile.flags & (HSX_TILE_CACHED | HSX_TILE_NT);
     a.nt = streams == HSX_TILE_NT || (streams == 0 && (BY == 16 || RING));
+    a.fold = MODE == MODE_AC2 ? t_fold.flags : 0;
+    for (int k = 0; k < 6; k++) a.fb[k] = a.fold ? (int)t_fold.fb[k] : -1;
     a.c = make_coefs<TC>(g);
     if (tiles * a.nchunk > (int64_t)1 << 30) return fail(HSX_ERR_UNSUPPORTED, "too many tiles");
     return HSX_OK;
 }
 
-template <typename TS, typename TC, int R, int VW, int LX, int BY, int MODE, bool RING>
+template <typename TS, typename TC, int R, int VW, int LX, int BY, int MODE, bool RING, bool FOLD = false>
 inline int launch_tiled(const hsx_grid *g, void *out, const void *prn_in, const void *u, const void *coef, hipStream_t s)
 {
     using K = Tiled<TS, TC, R, VW, LX, BY, MODE, RING>;
     static_assert(K::LDS <= TILE_LDS_MAX, "LDS tile above the workgroup limit");
-    auto kern = k_tiled<TS, TC, R, VW, LX, BY, MODE, RING>;
+    auto kern = k_tiled<TS, TC, R, VW, LX, BY, MODE, RING, FOLD>;
     static const int per_cu = tiled_per_cu(kern, LX * BY, K::LDS);
     TiledArgs<TC> a;
     int rc = fill_tiled_args<TS, TC, R, VW, LX, BY, MODE, RING>(g, out, prn_in, u, coef, (int64_t)per_cu * device_cus(), a);
@@ -572,7 +729,7 @@ struct TallFits {
                                  Tiled<TS, TC, R, VW, LX, 16, MODE, false>::LDS <= TILE_LDS_MAX;
 };
 
-template <typename TS, typename TC, int R, int VW, int LX, int MODE>
+template <typename TS, typename TC, int R, int VW, int LX, int MODE, bool FOLD = false>
 inline int tall_variant()
 {
     constexpr bool ring_ok = TallFits<TS, TC, R, VW, LX, MODE>::ring;
@@ -580,37 +737,37 @@ inline int tall_variant()
     const bool forced = g_tile.by == 16;
     if (g_tile.by == 8) return 0;
     if constexpr (ring_ok) {
-        static const bool ring_spills = tiled_spills(k_tiled<TS, TC, R, VW, LX, 16, MODE, true>);
+        static const bool ring_spills = tiled_spills(k_tiled<TS, TC, R, VW, LX, 16, MODE, true, FOLD>);
         if (!(g_tile.flags & HSX_TILE_NO_RING) && (forced || !ring_spills)) return 1;
     }
     if constexpr (tall_ok) {
-        static const bool tall_spills = tiled_spills(k_tiled<TS, TC, R, VW, LX, 16, MODE, false>);
+        static const bool tall_spills = tiled_spills(k_tiled<TS, TC, R, VW, LX, 16, MODE, false, FOLD>);
         if (forced || !tall_spills) return 2;
     }
     return 0;
 }
 
 // BY = 16 (with the halo ring where it fits) for tiles of at least 32 lanes, else 8
-template <typename TS, typename TC, int R, int VW, int LX, int MODE>
+template <typename TS, typename TC, int R, int VW, int LX, int MODE, bool FOLD = false>
 inline int launch_tiled_by(const hsx_grid *g, void *out, const void *prn_in, const void *u, const void *coef, hipStream_t s)
 {
-    const int tall = tall_variant<TS, TC, R, VW, LX, MODE>();
+    const int tall = tall_variant<TS, TC, R, VW, LX, MODE, FOLD>();
     if constexpr (TallFits<TS, TC, R, VW, LX, MODE>::ring) {
-        if (tall == 1) return launch_tiled<TS, TC, R, VW, LX, 16, MODE, true>(g, out, prn_in, u, coef, s);
+        if (tall == 1) return launch_tiled<TS, TC, R, VW, LX, 16, MODE, true, FOLD>(g, out, prn_in, u, coef, s);
     }
     if constexpr (TallFits<TS, TC, R, VW, LX, MODE>::tall) {
-        if (tall == 2) return launch_tiled<TS, TC, R, VW, LX, 16, MODE, false>(g, out, prn_in, u, coef, s);
+        if (tall == 2) return launch_tiled<TS, TC, R, VW, LX, 16, MODE, false, FOLD>(g, out, prn_in, u, coef, s);
     }
     // 8 rows: the ring only when 8 rows are asked for (at 1000^3 FP32 order 8
     // it measured 3.83 ms against 3.66 ms without it: its LDS halves the
     // resident workgroups while the 8-row halo stays large)
     constexpr bool ring8_ok = Tiled<TS, TC, R, VW, LX, 8, MODE, true>::RING &&
                               Tiled<TS, TC, R, VW, LX, 8, MODE, true>::LDS <= TILE_LDS_MAX;
-    if constexpr (ring8_ok) {
+    if constexpr (ring8_ok && !FOLD) {
         if (g_tile.by == 8 && !(g_tile.flags & HSX_TILE_NO_RING))
             return launch_tiled<TS, TC, R, VW, LX, 8, MODE, true>(g, out, prn_in, u, coef, s);
     }
-    return launch_tiled<TS, TC, R, VW, LX, 8, MODE, false>(g, out, prn_in, u, coef, s);
+    return launch_tiled<TS, TC, R, VW, LX, 8, MODE, false, FOLD>(g, out, prn_in, u, coef, s);
 }
 
 // alignment and size conditions of the tiled kernels
@@ -625,7 +782,7 @@ inline int tiled_fits(const hsx_grid *g, int vw_elems, size_t elem)
 // shape chosen by hsx_autotune for this grid and operation, if any
 bool tuned_shape(const hsx_grid *g, int mode, TileCfg *out);
 
-template <int MODE>
+template <int MODE, bool FOLD = false>
 inline int run_tiled_shape(const hsx_grid *g, void *out, const void *prn_in, const void *u, const void *coef,
                            hipStream_t s)
 {
@@ -641,18 +798,18 @@ inline int run_tiled_shape(const hsx_grid *g, void *out, const void *prn_in, con
         const int lx = pick_lx(g, VW);
         return dispatch_R(R, [&](auto rc_) -> int {
             constexpr int RR = decltype(rc_)::value;
-            if (lx == 16) return launch_tiled<TS, TC, RR, VW, 16, 8, MODE, false>(g, out, prn_in, u, coef, s);
+            if (lx == 16) return launch_tiled<TS, TC, RR, VW, 16, 8, MODE, false, FOLD>(g, out, prn_in, u, coef, s);
             // 64 lanes whose 16-row instances all spill (order 16 FP32): 32 lanes x 16 rows
-            const bool narrow = lx == 64 && g_tile.lx == 0 && tall_variant<TS, TC, RR, VW, 64, MODE>() == 0 &&
-                                tall_variant<TS, TC, RR, VW, 32, MODE>() != 0;
-            if (lx == 32 || narrow) return launch_tiled_by<TS, TC, RR, VW, 32, MODE>(g, out, prn_in, u, coef, s);
-            return launch_tiled_by<TS, TC, RR, VW, 64, MODE>(g, out, prn_in, u, coef, s);
+            const bool narrow = lx == 64 && g_tile.lx == 0 && tall_variant<TS, TC, RR, VW, 64, MODE, FOLD>() == 0 &&
+                                tall_variant<TS, TC, RR, VW, 32, MODE, FOLD>() != 0;
+            if (lx == 32 || narrow) return launch_tiled_by<TS, TC, RR, VW, 32, MODE, FOLD>(g, out, prn_in, u, coef, s);
+            return launch_tiled_by<TS, TC, RR, VW, 64, MODE, FOLD>(g, out, prn_in, u, coef, s);
         });
     });
 }
 
 // the forced shape (hsx_set_tile) if any, else the tuned one, else the defaults
-template <int MODE>
+template <int MODE, bool FOLD = false>
 inline int run_tiled(const hsx_grid *g, void *out, const void *prn_in, const void *u, const void *coef, hipStream_t s)
 {
     const TileCfg forced = g_tile;
@@ -660,7 +817,7 @@ inline int run_tiled(const hsx_grid *g, void *out, const void *prn_in, const voi
     const bool use_tuned = forced.lx == 0 && forced.by == 0 && forced.chunk == 0 && forced.flags == 0 &&
                            tuned_shape(g, MODE, &tuned);
     if (use_tuned) g_tile = tuned;
-    const int rc = run_tiled_shape<MODE>(g, out, prn_in, u, coef, s);
+    const int rc = run_tiled_shape<MODE, FOLD>(g, out, prn_in, u, coef, s);
     if (use_tuned) g_tile = forced;
     return rc;
 }

@@ -25,6 +25,8 @@ int set_tile(int lx, int by, int chunk, int flags)
     return HSX_OK;
 }
 
+thread_local FoldReq t_fold = {0, {-1, -1, -1, -1, -1, -1}};
+
 int tiled_ac2_step(const hsx_grid *g, void *out, const void *prn, const void *prc, const void *coef, hipStream_t s)
 {
     return run_tiled<MODE_AC2>(g, out, prn, prc, coef, s);

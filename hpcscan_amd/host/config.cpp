@@ -63,7 +63,7 @@ Config::Config(void)
 
     gpuDevice = 0;
     hipKernel = "auto";
-    hipFoldBC = true;
+    hipFoldBC = false;
 
     memset(hostName, 0, sizeof(hostName));
     memset(userName, 0, sizeof(userName));
@@ -146,8 +146,8 @@ std::vector<Flag> flagTable(Config &c)
         // MI355X backend options
         {"-gpuDevice", K::Int, &c.gpuDevice, " GPU device\t", C::NonNeg, "MI355X: HIP device ordinal"},
         {"-hipKernel", K::Str, &c.hipKernel, " HIP kernel\t", C::None, "MI355X: stencil kernels auto (DEFAULT) | ref | tiled"},
-        {"-hipFoldBC", K::On, &c.hipFoldBC, " HIP fold BC\t", C::None, "MI355X: fold the free surface into the fused step (DEFAULT)"},
-        {"-hipNoFoldBC", K::Off, &c.hipFoldBC, " HIP fold BC\t", C::None, "MI355X: apply the free surface as a separate pass"},
+        {"-hipFoldBC", K::On, &c.hipFoldBC, " HIP fold BC\t", C::None, "MI355X: fold the free surface into the fused step"},
+        {"-hipNoFoldBC", K::Off, &c.hipFoldBC, " HIP fold BC\t", C::None, "MI355X: apply the free surface as a separate pass (DEFAULT)"},
     };
 }
 

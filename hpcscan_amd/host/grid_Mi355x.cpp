@@ -117,11 +117,22 @@ const Grid_Mi355x *Grid_Mi355x::peer(const Grid &g, const char *what) const
         printError(m.c_str());
         return nullptr;
     }
-    return p;
+    return p->settle() ? p : nullptr;
+}
+
+bool Grid_Mi355x::settle(void) const
+{
+    if (!fsPending) return true;
+    const int mask = fsPending;
+    fsPending = 0;
+    hsx_grid g = desc();
+    return check(hsx_free_surface(&g, d_grid, mask, nullptr), "hsx_free_surface") &&
+           check(hsx_stream_sync(nullptr), "hsx_stream_sync");
 }
 
 Rtn_code Grid_Mi355x::toHost(void)
 {
+    if (!settle()) return RTN_CODE_KO;
     return check(hsx_memcpy_d2h(grid_3d, d_grid, npoint * sizeof(Myfloat), nullptr), "hsx_memcpy_d2h")
                ? RTN_CODE_OK
                : RTN_CODE_KO;
@@ -155,6 +166,8 @@ void Grid_Mi355x::fill(Point_type p, Myfloat val)
 {
     int64_t r[6];
     range(p, r);
+    if (!settle()) return;
+    touched();
     hsx_grid g = desc();
     if (check(hsx_fill_const(&g, d_grid, r, val, nullptr), "hsx_fill_const"))
         check(hsx_stream_sync(nullptr), "hsx_stream_sync");
@@ -186,6 +199,8 @@ void Grid_Mi355x::fill(Point_type p, Func_type t1, Func_type t2, Func_type t3, M
             tab[a].push_back(v);
         }
     }
+    if (!settle()) return;
+    touched();
     hsx_grid g = desc();
     check(hsx_fill_separable(&g, d_grid, r, tab[0].data(), tab[1].data(), tab[2].data(), amp, nullptr),
           "hsx_fill_separable");
@@ -193,6 +208,8 @@ void Grid_Mi355x::fill(Point_type p, Func_type t1, Func_type t2, Func_type t3, M
 
 void Grid_Mi355x::fillArray(Myfloat val)
 {
+    if (!settle()) return;
+    touched();
     hsx_grid g = desc();
     if (check(hsx_array_op(&g, HSX_ARRAY_FILL, d_grid, nullptr, nullptr, val, nullptr), "hsx_array_op"))
         check(hsx_stream_sync(nullptr), "hsx_stream_sync");
@@ -201,7 +218,8 @@ void Grid_Mi355x::fillArray(Myfloat val)
 void Grid_Mi355x::copyArray(const Grid &u)
 {
     const Grid_Mi355x *pu = peer(u, "copyArray");
-    if (!pu) return;
+    if (!pu || !settle()) return;
+    touched();
     hsx_grid g = desc();
     if (check(hsx_array_op(&g, HSX_ARRAY_COPY, d_grid, pu->d_grid, nullptr, 0, nullptr), "hsx_array_op"))
         check(hsx_stream_sync(nullptr), "hsx_stream_sync");
@@ -210,7 +228,8 @@ void Grid_Mi355x::copyArray(const Grid &u)
 void Grid_Mi355x::addArray(const Grid &u, const Grid &v)
 {
     const Grid_Mi355x *pu = peer(u, "addArray"), *pv = peer(v, "addArray");
-    if (!pu || !pv) return;
+    if (!pu || !pv || !settle()) return;
+    touched();
     hsx_grid g = desc();
     if (check(hsx_array_op(&g, HSX_ARRAY_ADD, d_grid, pu->d_grid, pv->d_grid, 0, nullptr), "hsx_array_op"))
         check(hsx_stream_sync(nullptr), "hsx_stream_sync");
@@ -219,7 +238,8 @@ void Grid_Mi355x::addArray(const Grid &u, const Grid &v)
 void Grid_Mi355x::multiplyArray(const Grid &u, const Grid &v)
 {
     const Grid_Mi355x *pu = peer(u, "multiplyArray"), *pv = peer(v, "multiplyArray");
-    if (!pu || !pv) return;
+    if (!pu || !pv || !settle()) return;
+    touched();
     hsx_grid g = desc();
     if (check(hsx_array_op(&g, HSX_ARRAY_MULTIPLY, d_grid, pu->d_grid, pv->d_grid, 0, nullptr), "hsx_array_op"))
         check(hsx_stream_sync(nullptr), "hsx_stream_sync");
@@ -228,7 +248,8 @@ void Grid_Mi355x::multiplyArray(const Grid &u, const Grid &v)
 void Grid_Mi355x::addUpdateArray(const Grid &u)
 {
     const Grid_Mi355x *pu = peer(u, "addUpdateArray");
-    if (!pu) return;
+    if (!pu || !settle()) return;
+    touched();
     hsx_grid g = desc();
     if (check(hsx_array_op(&g, HSX_ARRAY_ADDTO, d_grid, pu->d_grid, nullptr, 0, nullptr), "hsx_array_op"))
         check(hsx_stream_sync(nullptr), "hsx_stream_sync");
@@ -247,8 +268,17 @@ Rtn_code Grid_Mi355x::applyBoundaryCondition(BoundCond_type bc)
     int mask = 0;
     for (int k = 0; k < 2 * (int)dim; k++)
         if (getNeighbourProc(halos[k]) == PROC_NULL) mask |= 1 << k;
+    // halos already hold the free surface of the inner points (written by the
+    // folded step that produced them): the zeros of the boundary planes are
+    // owed to the next step, which reads them as zero
+    if (Config::Instance()->hipFoldBC && dim == DIM3 && mask != 0 && fsHalo == mask) {
+        fsPending = mask;
+        return RTN_CODE_OK;
+    }
+    if (!settle()) return RTN_CODE_KO;
     hsx_grid g = desc();
     if (!check(hsx_free_surface(&g, d_grid, mask, nullptr), "hsx_free_surface")) return RTN_CODE_KO;
+    fsHalo = mask;
     return check(hsx_stream_sync(nullptr), "hsx_stream_sync") ? RTN_CODE_OK : RTN_CODE_KO;
 }
 
@@ -266,7 +296,8 @@ Rtn_code Grid_Mi355x::fdD2(int axis, const Grid &W, Myint fdOrder)
         return RTN_CODE_KO;
     }
     const Grid_Mi355x *pw = peer(W, "FD_D2");
-    if (!pw) return RTN_CODE_KO;
+    if (!pw || !settle()) return RTN_CODE_KO;
+    pw->touched();
     hsx_grid g = desc();
     if (!check(hsx_fd_d2(&g, axis, d_grid, pw->d_grid, nullptr), "hsx_fd_d2")) return RTN_CODE_KO;
     return check(hsx_stream_sync(nullptr), "hsx_stream_sync") ? RTN_CODE_OK : RTN_CODE_KO;
@@ -287,7 +318,8 @@ Rtn_code Grid_Mi355x::FD_LAPLACIAN(Point_type, const Grid &W, Myint fdOrder)
         return RTN_CODE_KO;
     }
     const Grid_Mi355x *pw = peer(W, "FD_LAPLACIAN");
-    if (!pw) return RTN_CODE_KO;
+    if (!pw || !settle()) return RTN_CODE_KO;
+    pw->touched();
     hsx_grid g = desc();
     if (!check(hsx_laplacian(&g, d_grid, pw->d_grid, nullptr), "hsx_laplacian")) return RTN_CODE_KO;
     return check(hsx_stream_sync(nullptr), "hsx_stream_sync") ? RTN_CODE_OK : RTN_CODE_KO;
@@ -301,7 +333,8 @@ Rtn_code Grid_Mi355x::updatePressure(Point_type, const Grid &prc, const Grid &co
     }
     const Grid_Mi355x *pp = peer(prc, "updatePressure"), *pc = peer(coef, "updatePressure"),
                       *pl = peer(lapla, "updatePressure");
-    if (!pp || !pc || !pl) return RTN_CODE_KO;
+    if (!pp || !pc || !pl || !settle()) return RTN_CODE_KO;
+    touched();
     hsx_grid g = desc();
     if (!check(hsx_update_pressure(&g, d_grid, pp->d_grid, pc->d_grid, pl->d_grid, nullptr), "hsx_update_pressure"))
         return RTN_CODE_KO;
@@ -318,9 +351,29 @@ Rtn_code Grid_Mi355x::computePressureWithFD(Grid &prc, Grid &coef, Myint fdOrder
         printError(" In Grid_Mi355x, fdOrder does not match the grid halo width", fdOrder);
         return RTN_CODE_KO;
     }
-    const Grid_Mi355x *pp = peer(prc, "computePressureWithFD"), *pc = peer(coef, "computePressureWithFD");
-    if (!pp || !pc) return RTN_CODE_KO;
+    const Grid_Mi355x *pp = dynamic_cast<const Grid_Mi355x *>(&prc);
+    const Grid_Mi355x *pc = peer(coef, "computePressureWithFD");
+    if (!pp || !pp->d_grid || !pc) {
+        printError(" In Grid_Mi355x::computePressureWithFD, argument grid is not an initialized MI355X grid");
+        return RTN_CODE_KO;
+    }
     hsx_grid g = desc();
+    // free surface of prc (owed, or applied with its halos) folded into the step
+    const int mask = pp->fsPending ? pp->fsPending : pp->fsHalo;
+    if (mask && Config::Instance()->hipFoldBC) {
+        if (fsPending && fsPending != mask && !settle()) return RTN_CODE_KO;
+        const int fold = HSX_FOLD_HALO | (pp->fsPending ? HSX_FOLD_PRC : 0) | (fsPending ? HSX_FOLD_PRN : 0);
+        const int rc = hsx_ac2_step_fs(&g, d_grid, pp->d_grid, pc->d_grid, mask, fold, nullptr);
+        if (rc == HSX_OK) {
+            fsPending = 0;
+            fsHalo = mask;
+            return check(hsx_stream_sync(nullptr), "hsx_stream_sync") ? RTN_CODE_OK : RTN_CODE_KO;
+        }
+        if (rc != HSX_ERR_UNSUPPORTED) return check(rc, "hsx_ac2_step_fs") ? RTN_CODE_OK : RTN_CODE_KO;
+    }
+    // grids the folded kernel does not take: free surface and step apart
+    if (!pp->settle() || !settle()) return RTN_CODE_KO;
+    touched();
     if (!check(hsx_ac2_step(&g, d_grid, pp->d_grid, pc->d_grid, nullptr), "hsx_ac2_step")) return RTN_CODE_KO;
     return check(hsx_stream_sync(nullptr), "hsx_stream_sync") ? RTN_CODE_OK : RTN_CODE_KO;
 }
@@ -336,6 +389,7 @@ Myfloat Grid_Mi355x::L1Err(Point_type p, const Grid &v) const
     if (!pv) return -1.0;
     int64_t r[6];
     range(p, r);
+    if (!settle()) return NAN;
     hsx_grid g = desc();
     double s[2];
     if (!check(hsx_l1_sums(&g, d_grid, pv->d_grid, r, s, nullptr), "hsx_l1_sums")) return NAN;
@@ -355,6 +409,7 @@ Myfloat Grid_Mi355x::maxErr(Point_type p, const Grid &v) const
     if (!pv) return -1.0;
     int64_t r[6];
     range(p, r);
+    if (!settle()) return NAN;
     hsx_grid g = desc();
     double e;
     if (!check(hsx_max_err(&g, d_grid, pv->d_grid, r, MAX_ERR_FLOAT, &e, nullptr), "hsx_max_err")) return NAN;
@@ -365,6 +420,7 @@ Myfloat Grid_Mi355x::getSumAbs(Point_type p) const
 {
     int64_t r[6];
     range(p, r);
+    if (!settle()) return NAN;
     hsx_grid g = desc();
     double s;
     if (!check(hsx_sum_abs(&g, d_grid, r, &s, nullptr), "hsx_sum_abs")) return NAN;
@@ -382,6 +438,7 @@ Myfloat Grid_Mi355x::getSumAbsDiff(Point_type p, const Grid &v) const
     if (!pv) return -1.0;
     int64_t r[6];
     range(p, r);
+    if (!settle()) return NAN;
     hsx_grid g = desc();
     double s;
     if (!check(hsx_sum_abs_diff(&g, d_grid, pv->d_grid, r, &s, nullptr), "hsx_sum_abs_diff")) return NAN;
@@ -393,6 +450,7 @@ Myfloat Grid_Mi355x::getMin(Point_type p)
 {
     int64_t r[6];
     range(p, r);
+    if (!settle()) return NAN;
     hsx_grid g = desc();
     double mm[2];
     if (!check(hsx_min_max(&g, d_grid, r, mm, nullptr), "hsx_min_max")) return NAN;
@@ -403,6 +461,7 @@ Myfloat Grid_Mi355x::getMax(Point_type p)
 {
     int64_t r[6];
     range(p, r);
+    if (!settle()) return NAN;
     hsx_grid g = desc();
     double mm[2];
     if (!check(hsx_min_max(&g, d_grid, r, mm, nullptr), "hsx_min_max")) return NAN;

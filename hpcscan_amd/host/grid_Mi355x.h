@@ -52,6 +52,16 @@ public:
     void *d_grid;
 
 protected:
+    // Free surface folded into the next step.  applyBoundaryCondition on a grid
+    // whose halos already hold the free-surface values of its inner points only
+    // records the zeros it owes on the boundary planes (fsPending = face mask);
+    // computePressureWithFD reads those planes as zero (hsx_ac2_step_fs) and
+    // writes the new grid's halos (fsHalo).  Any other use of a grid first
+    // applies what it owes (settle).
+    mutable int fsPending = 0;
+    mutable int fsHalo = 0;
+    bool settle(void) const;
+    void touched(void) const { fsHalo = 0; }
     hsx_grid desc(void) const;
     void range(Point_type, int64_t r[6]) const;
     bool check(int rc, const char *what) const;

@@ -154,6 +154,29 @@ int hsx_ac2_step(const hsx_grid *g, void *prn, const void *prc, const void *coef
  * recomputation of a checkpoint segment).  No counterpart in the reference. */
 int hsx_ac2_step_to(const hsx_grid *g, void *out, const void *prn, const void *prc, const void *coef,
                     void *stream);
+/* Propagator_Ac2::computeWavefieldNextTimeStep's two device calls in one launch
+ * (src/propagator_Ac2.cpp:366-393: applyBoundaryCondition(FREESURF) on prc, then
+ * computePressureWithFD), with the free surface on the faces of face_mask folded
+ * into the step.  fold:
+ *   HSX_FOLD_PRC   prc is read as the free surface leaves it: its inner points on
+ *                  the boundary planes read as 0.  Its halos must already hold the
+ *                  free-surface values of its inner points (written by the previous
+ *                  folded step with HSX_FOLD_HALO, or by hsx_free_surface).
+ *   HSX_FOLD_PRN   the same for prn (the grid the free surface was applied to one
+ *                  step earlier, still only logically).
+ *   HSX_FOLD_HALO  after the step, prn's halos on the faces of face_mask hold the
+ *                  free-surface values of its new inner points (its boundary planes
+ *                  keep the computed values, as after the reference step).
+ * Inner results equal hsx_free_surface(prc) [+ hsx_free_surface(prn)] followed by
+ * hsx_ac2_step bit for bit; prc is not written.  Returns HSX_ERR_UNSUPPORTED when
+ * the grid does not take the folded kernel (not 3D, fewer than R+2 inner points on
+ * a folded axis, reference kernels forced): the caller then applies the free
+ * surface and steps separately. */
+#define HSX_FOLD_PRC 1
+#define HSX_FOLD_PRN 2
+#define HSX_FOLD_HALO 4
+int hsx_ac2_step_fs(const hsx_grid *g, void *prn, const void *prc, const void *coef, int face_mask, int fold,
+                    void *stream);
 
 /* ---- reductions: Grid::getSumAbs ... (src/grid.cpp:2006-2370) -------------
  * All accumulate in double on the device; the result is returned on the host. */

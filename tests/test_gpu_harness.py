@@ -65,6 +65,26 @@ def test_same_report_as_cpu_reference(tmp_path, prec, args):
 
 
 @pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("args", [
+    ["-testCase", "Propa", "-dim", "3", "-fdOrder", "8"],
+    ["-testCase", "Propa", "-dim", "3", "-fdOrder", "4", "-n1", "72", "-n2", "40", "-n3", "36", "-nt", "40"],
+    ["-testCase", "Propa", "-dim", "3", "-fdOrder", "16", "-nt", "30"],
+    ["-testCase", "Propa", "-dim", "3", "-fdOrder", "8", "-n1", "100", "-n2", "100", "-n3", "100", "-nt", "60",
+     "-param1", "1", "-param2", "1", "-param3", "1", "-snapInc", "7"],
+    ["-testCase", "Propa", "-dim", "2", "-fdOrder", "8"],          # 2D: the free surface stays apart
+])
+def test_folded_free_surface_same_report(tmp_path, prec, args):
+    """-hipFoldBC (free surface folded into the fused step, its boundary-plane
+    zeros owed until the grid is used otherwise) prints the CPU run's errors."""
+    print(" ".join(args), prec, flush=True)
+    rc1, gpu = run_bin(binary(prec), args + ["-testMode", "MI355X", "-ntry", "2", "-hipFoldBC"], tmp_path)
+    rc2, cpu = run_bin(CPU_BIN[prec], args + ["-testMode", "CacheBlk", "-ntry", "1"], tmp_path)
+    assert rc1 == 0 and rc2 == 0
+    assert _errors(gpu) == _errors(cpu)
+    assert ">> Test FAILED" not in gpu and "F A T A L" not in gpu
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
 @pytest.mark.parametrize("dim", ["1", "2", "3"])
 def test_grid_test_case_on_gpu(tmp_path, prec, dim):
     """-testCase Grid on the MI355X backend: the nine operations pass their

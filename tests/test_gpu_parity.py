@@ -395,3 +395,74 @@ def test_tiled_rejects_misaligned_rows(capi):
             capi.laplacian(g, d, capi.DeviceArray(p.npoint * 4))
     finally:
         capi.set_kernel(capi.HSX_KERNEL_AUTO)
+
+
+FOLD_CASES = [
+    # (order, prec, n1 target, n2, n3, face mask, lanes, chunk, rows, flags)
+    (8, "f32", 72, 40, 36, 0x3F, 0, 0, 0, 0),        # default tile (16 rows, ring)
+    (8, "f32", 72, 40, 36, 0x16, 16, 5, 8, 0),       # some faces only, several chunks per n3 column
+    (8, "f32", 200, 37, 31, 0x3F, 64, 7, 16, 4),     # 16 rows without the ring
+    (8, "f32", 130, 19, 30, 0x29, 32, 0, 8, 1),      # 8 rows with the ring, cached streams
+    (16, "f32", 140, 30, 24, 0x3F, 32, 0, 16, 0),    # R = 8
+    (16, "f32", 68, 25, 19, 0x3F, 16, 6, 0, 0),      # R = 8, 8 rows: halo rows of the second tile row are inner
+    (2, "f32", 69, 20, 22, 0x3F, 0, 0, 0, 0),        # R = 1: the first lane holds halo and inner columns
+    (6, "f32", 69, 19, 23, 0x3F, 16, 4, 0, 0),       # R = 3, n1 offset
+    (8, "f32", 72, 6, 6, 0x3F, 0, 0, 0, 0),          # N2 = N3 = R + 2: both mirrors of a plane meet
+    (4, "f64", 70, 21, 27, 0x3F, 0, 0, 0, 0),
+    (12, "f64", 66, 30, 26, 0x1B, 32, 11, 0, 1),
+    (8, "f32c64", 72, 18, 21, 0x3F, 0, 0, 0, 0),
+]
+
+
+@pytest.mark.parametrize("order,prec,n1,n2,n3,mask,lanes,chunk,rows,flags", FOLD_CASES)
+def test_folded_free_surface_bit_exact(oracle, capi, order, prec, n1, n2, n3, mask, lanes, chunk, rows, flags):
+    """hsx_ac2_step_fs over 6 steps: every new wavefield equals the oracle's free
+    surface + fused step (inner points, bit for bit), and once the pending free
+    surface is applied, whole grids (halos included) equal the oracle's."""
+    dtype, c64 = PRECS[prec]
+    vw = 16 // np.dtype(dtype).itemsize
+    n = (_n1_aligned(n1, order, dtype, vw), n2, n3)
+    print(f"fold order={order} prec={prec} n={n} mask={mask:#x} lanes={lanes} chunk={chunk} rows={rows} "
+          f"flags={flags}", flush=True)
+    p, o = _layouts(oracle, 3, n, order, dtype)
+    rng = np.random.default_rng(order * 11 + n3)
+    prc, prn = _rand(rng, p.npoint, dtype), _rand(rng, p.npoint, dtype)
+    coef = (0.05 * (1 + 0.5 * rng.random(p.npoint))).astype(dtype)
+    iv = oracle.inv2(p.d, dtype, compute64=c64)
+    g = capi.hsx_grid(p, compute_fp64=c64)
+    dprc, dprn, dcoef = (capi.DeviceArray.from_numpy(x) for x in (prc, prn, coef))
+    a, b = prn.copy(), prc.copy()
+    capi.set_tile(lanes, chunk, rows, flags)
+    try:
+        capi.free_surface(g, dprc, face_mask=mask)       # the first free surface is applied, later ones folded
+        for it in range(6):
+            oracle.free_surface(o, b, compute64=c64, face_mask=mask)
+            oracle.pressure_fd(o, a, b, coef, iv, compute64=c64)
+            fold = capi.HSX_FOLD_PRC | capi.HSX_FOLD_HALO | (capi.HSX_FOLD_PRN if it >= 1 else 0)
+            capi.ac2_step_fs(g, dprn, dprc, dcoef, fold, face_mask=mask)
+            got = dprn.download(dtype)
+            box = p.inner()
+            sl = (slice(box[4], box[5] + 1), slice(box[2], box[3] + 1), slice(box[0], box[1] + 1))
+            shape = (p.n3, p.n2, p.n1)
+            _same(got.reshape(shape)[sl], a.reshape(shape)[sl])
+            a, b = b, a
+            dprn, dprc = dprc, dprn
+        # apply the pending free surface of both grids, then compare everything
+        oracle.free_surface(o, b, compute64=c64, face_mask=mask)
+        capi.free_surface(g, dprc, face_mask=mask)
+        capi.free_surface(g, dprn, face_mask=mask)
+        assert np.array_equal(dprc.download(dtype).view(np.uint8), b.view(np.uint8))
+        assert np.array_equal(dprn.download(dtype).view(np.uint8), a.view(np.uint8))
+    finally:
+        capi.set_tile(0, 0)
+
+
+def test_folded_free_surface_refuses_small_axes(capi):
+    """Fewer than R+2 inner points on a folded axis: no folded kernel."""
+    p = make_layout(3, (72, 40, 5), 8)
+    g = capi.hsx_grid(p)
+    d = [capi.DeviceArray(p.npoint * 4) for _ in range(3)]
+    with pytest.raises(capi.HsxError) as e:
+        capi.ac2_step_fs(g, d[0], d[1], d[2], capi.HSX_FOLD_PRC)
+    assert e.value.rc == capi.HSX_ERR_UNSUPPORTED
+    capi.ac2_step_fs(g, d[0], d[1], d[2], capi.HSX_FOLD_PRC, face_mask=0x0F)   # n3 faces not folded
