@@ -96,6 +96,7 @@ SIGNATURES = {
     "hsx_min_max": (_i, [_G, _vp, _R, _P(_d), _vp]),
     "hsx_ac2_step_to": (_i, [_G, _vp, _vp, _vp, _vp, _vp]),
     "hsx_ac2_step_fs": (_i, [_G, _vp, _vp, _vp, _i, _i, _vp]),
+    "hsx_fd_d1": (_i, [_G, _i, _vp, _vp, _vp]),
     "hsx_inject": (_i, [_G, _vp, _i64, _vp, _vp, _vp, _d, _i, _vp]),
     "hsx_gather": (_i, [_G, _vp, _i64, _vp, _vp, _vp]),
     "hsx_imaging": (_i, [_G, _vp, _vp, _vp, _vp, _vp, _R, _d, _i64, _d, _vp]),
@@ -283,6 +284,10 @@ def free_surface(g: HsxGrid, u, face_mask=ALL_FACES, stream=None):
 
 def fd_d2(g: HsxGrid, axis, u, w, stream=None):
     check(lib().hsx_fd_d2(ctypes.byref(g), axis, dptr(u), dptr(w), stream))
+
+
+def fd_d1(g: HsxGrid, axis, u, w, stream=None):
+    check(lib().hsx_fd_d1(ctypes.byref(g), axis, dptr(u), dptr(w), stream))
 
 
 def laplacian(g: HsxGrid, u, w, stream=None):

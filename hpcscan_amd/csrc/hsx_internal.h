@@ -77,17 +77,22 @@ template <typename TC>
 struct Coefs {
     TC A[9];        // A0..AR
     TC inv2[3];     // (TC)(1/d) squared in TC
+    TC D1[9];       // staggered first-derivative weights, D1[1..R]
+    TC inv[3];      // (TC)(1/d)
 };
 double fd_coef_double(int fd_order, int j);
+double fd1_coef_double(int fd_order, int j);
 template <typename TC>
 inline Coefs<TC> make_coefs(const hsx_grid *g)
 {
     Coefs<TC> c;
     const int R = g->fd_order / 2;
     for (int j = 0; j < 9; j++) c.A[j] = j <= R ? (TC)fd_coef_double(g->fd_order, j) : (TC)0;
+    for (int j = 0; j < 9; j++) c.D1[j] = j >= 1 && j <= R ? (TC)fd1_coef_double(g->fd_order, j) : (TC)0;
     for (int a = 0; a < 3; a++) {
         const TC inv = (TC)(1.0 / g->d[a]);
         c.inv2[a] = inv * inv;
+        c.inv[a] = inv;
     }
     return c;
 }
@@ -147,6 +152,13 @@ int tiled_fd_d2_z(const hsx_grid *g, const void *u, void *w, hipStream_t s);
 inline int tiled_fd_d2(const hsx_grid *g, int axis, const void *u, void *w, hipStream_t s)
 {
     return axis == 0 ? tiled_fd_d2_x(g, u, w, s) : axis == 1 ? tiled_fd_d2_y(g, u, w, s) : tiled_fd_d2_z(g, u, w, s);
+}
+int tiled_fd_d1_x(const hsx_grid *g, const void *u, void *w, hipStream_t s);
+int tiled_fd_d1_y(const hsx_grid *g, const void *u, void *w, hipStream_t s);
+int tiled_fd_d1_z(const hsx_grid *g, const void *u, void *w, hipStream_t s);
+inline int tiled_fd_d1(const hsx_grid *g, int axis, const void *u, void *w, hipStream_t s)
+{
+    return axis == 0 ? tiled_fd_d1_x(g, u, w, s) : axis == 1 ? tiled_fd_d1_y(g, u, w, s) : tiled_fd_d1_z(g, u, w, s);
 }
 int set_tile(int lx, int by, int chunk, int streams);
 

@@ -166,6 +166,27 @@ double fd_coef_double(int fd_order, int j)
     return q[0] / q[1];
 }
 
+// staggered first-derivative weights A1..AR (src/fdm.h:29-268), index 0 unused
+static const double FD1_NUM_DEN[8][9][2] = {
+    {{0, 1}, {1, 1}},
+    {{0, 1}, {9, 8}, {-1, 24}},
+    {{0, 1}, {75, 64}, {-25, 384}, {3, 640}},
+    {{0, 1}, {1225, 1024}, {-245, 3072}, {49, 5120}, {-5, 7168}},
+    {{0, 1}, {19845, 16384}, {-735, 8192}, {567, 40960}, {-405, 229376}, {35, 294912}},
+    {{0, 1}, {160083, 131072}, {-12705, 131072}, {22869, 1310720}, {-5445, 1835008}, {847, 2359296},
+     {-63, 2883584}},
+    {{0, 1}, {1288287, 1048576}, {-429429, 4194304}, {429429, 20971520}, {-61347, 14680064}, {13013, 18874368},
+     {-3549, 46137344}, {231, 54525952}},
+    {{0, 1}, {41409225, 33554432}, {-3578575, 33554432}, {3864861, 167772160}, {-1254825, 234881024},
+     {325325, 301989888}, {-61425, 369098752}, {7425, 436207616}, {-143, 167772160}},
+};
+
+double fd1_coef_double(int fd_order, int j)
+{
+    const double *q = FD1_NUM_DEN[fd_order / 2 - 1][j];
+    return q[1] != 0 ? q[0] / q[1] : 0.0;
+}
+
 } // namespace hsx
 
 using namespace hsx;

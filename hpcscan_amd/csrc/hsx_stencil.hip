@@ -28,6 +28,19 @@ __global__ void __launch_bounds__(RX *RY) k_fd_d2_ref(const TS *__restrict__ u, 
     w[ii] = (TS)d2_point<TS, TC, R>(u, ii, stride, c.A, inv2);
 }
 
+template <typename TS, typename TC, int R>
+__global__ void __launch_bounds__(RX *RY) k_fd_d1_ref(const TS *__restrict__ u, TS *__restrict__ w, Box b,
+                                                      int64_t n1, int64_t n12, int64_t stride, Coefs<TC> c,
+                                                      TC inv)
+{
+    const int64_t i1 = b.s1 + (int64_t)blockIdx.x * RX + threadIdx.x;
+    const int64_t i2 = b.s2 + (int64_t)blockIdx.y * RY + threadIdx.y;
+    const int64_t i3 = b.s3 + blockIdx.z;
+    if (i1 > b.e1 || i2 > b.e2) return;
+    const int64_t ii = i1 + i2 * n1 + i3 * n12;
+    w[ii] = (TS)d1_point<TS, TC, R>(u, ii, stride, c.D1, inv);
+}
+
 template <typename TS, typename TC, int R, int DIM>
 __global__ void __launch_bounds__(RX *RY) k_laplacian_ref(const TS *__restrict__ u, TS *__restrict__ w, Box b,
                                                           int64_t n1, int64_t n12, Coefs<TC> c)
@@ -100,6 +113,34 @@ int hsx_fd_d2(const hsx_grid *g, int axis, const void *u, void *w, void *stream)
             hipLaunchKernelGGL((k_fd_d2_ref<TS, TC, R>), grid, dim3(RX, RY), 0, s, (const TS *)u, (TS *)w, b,
                                g->n1, g->n1 * g->n2, stride, c, c.inv2[axis]);
             HSX_LAUNCHED("k_fd_d2_ref");
+            return HSX_OK;
+        });
+    });
+}
+
+int hsx_fd_d1(const hsx_grid *g, int axis, const void *u, void *w, void *stream)
+{
+    int rc = stencil_checks(g, {{u, "u"}, {w, "w"}});
+    if (rc) return rc;
+    if (axis < 0 || axis >= g->dim) return fail(HSX_ERR_INVALID, "axis %d with dim %d", axis, g->dim);
+    hipStream_t s = stream_of(stream);
+    if (kernel_family() != HSX_KERNEL_REF) {
+        rc = tiled_fd_d1(g, axis, u, w, s);
+        if (rc != HSX_ERR_UNSUPPORTED || kernel_family() == HSX_KERNEL_TILED) return rc;
+    }
+    const Box b = box_of(g->inner);
+    dim3 grid;
+    if ((rc = ref_grid(b, grid))) return rc;
+    const int64_t stride = axis == 0 ? 1 : axis == 1 ? g->n1 : g->n1 * g->n2;
+    return dispatch_types(g, [&](auto ts, auto tc) -> int {
+        using TS = decltype(ts);
+        using TC = decltype(tc);
+        const Coefs<TC> c = make_coefs<TC>(g);
+        return dispatch_R(g->fd_order / 2, [&](auto rc_) -> int {
+            constexpr int R = decltype(rc_)::value;
+            hipLaunchKernelGGL((k_fd_d1_ref<TS, TC, R>), grid, dim3(RX, RY), 0, s, (const TS *)u, (TS *)w, b,
+                               g->n1, g->n1 * g->n2, stride, c, c.inv[axis]);
+            HSX_LAUNCHED("k_fd_d1_ref");
             return HSX_OK;
         });
     });

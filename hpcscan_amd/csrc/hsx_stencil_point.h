@@ -48,6 +48,35 @@ __device__ __forceinline__ TC d2_values(TS v0, const TS *vp, const TS *vm, const
     return acc * inv2;
 }
 
+// staggered D1 from gathered values: vp[j-1] at +(j-1), vm[j-1] at -j, i.e. the
+// derivative at i - 1/2: (A1*(u0 - u-1) + A2*(u+1 - u-2) + ...) * inv, each
+// difference in the storage type, left to right (src/fdm.h:29-268); order 2
+// has no weight: (u0 - u-1) * inv
+template <typename TS, typename TC, int R>
+__device__ __forceinline__ TC d1_values(const TS *vp, const TS *vm, const TC *D1, TC inv)
+{
+    if (R == 1) return (TC)(TS)(vp[0] - vm[0]) * inv;
+    TC acc = D1[1] * (TC)(TS)(vp[0] - vm[0]);
+#pragma unroll
+    for (int j = 2; j <= R; j++) {
+        const TS diff = vp[j - 1] - vm[j - 1];
+        acc = acc + D1[j] * (TC)diff;
+    }
+    return acc * inv;
+}
+
+template <typename TS, typename TC, int R>
+__device__ __forceinline__ TC d1_point(const TS *__restrict__ u, int64_t c, int64_t s, const TC *D1, TC inv)
+{
+    TS vp[R], vm[R];
+#pragma unroll
+    for (int j = 1; j <= R; j++) {
+        vp[j - 1] = u[c + (j - 1) * s];
+        vm[j - 1] = u[c - j * s];
+    }
+    return d1_values<TS, TC, R>(vp, vm, D1, inv);
+}
+
 // Laplacian: N1 (+ N2 (+ N3)) summed left to right in the compute type
 template <typename TS, typename TC, int R, int DIM>
 __device__ __forceinline__ TC laplacian_point(const TS *__restrict__ u, int64_t ii, int64_t n1, int64_t n12,

@@ -28,7 +28,10 @@
 
 namespace hsx {
 
-enum TiledMode { MODE_AC2 = 0, MODE_LAP = 1, MODE_D2X = 2, MODE_D2Y = 3, MODE_D2Z = 4 };
+enum TiledMode {
+    MODE_AC2 = 0, MODE_LAP = 1, MODE_D2X = 2, MODE_D2Y = 3, MODE_D2Z = 4,
+    MODE_D1X = 5, MODE_D1Y = 6, MODE_D1Z = 7        // staggered first derivatives
+};
 
 template <typename T, int VW>
 struct alignas(sizeof(T) * VW) Pack {
@@ -69,9 +72,10 @@ struct Tiled {
     static constexpr int PLANE = ROWS * W;            // elements of one LDS plane
     static constexpr int NHG = HL / VW;               // halo groups per side
     static constexpr int NHR = ceil_div(2 * R, BY);   // n2 halo rows loaded per thread
-    static constexpr bool NEED_X = MODE == MODE_AC2 || MODE == MODE_LAP || MODE == MODE_D2X;
-    static constexpr bool NEED_Y = MODE == MODE_AC2 || MODE == MODE_LAP || MODE == MODE_D2Y;
-    static constexpr bool NEED_Z = MODE == MODE_AC2 || MODE == MODE_LAP || MODE == MODE_D2Z;
+    static constexpr bool NEED_X = MODE == MODE_AC2 || MODE == MODE_LAP || MODE == MODE_D2X || MODE == MODE_D1X;
+    static constexpr bool NEED_Y = MODE == MODE_AC2 || MODE == MODE_LAP || MODE == MODE_D2Y || MODE == MODE_D1Y;
+    static constexpr bool NEED_Z = MODE == MODE_AC2 || MODE == MODE_LAP || MODE == MODE_D2Z || MODE == MODE_D1Z;
+    static constexpr bool D1 = MODE == MODE_D1X || MODE == MODE_D1Y || MODE == MODE_D1Z;
     static constexpr bool NEED_LDS = NEED_X || NEED_Y;
     using V = Pack<TS, VW>;
     // Halo ring: the n1 / n2 halo of plane z+R is loaded together with the
@@ -479,7 +483,25 @@ __device__ __forceinline__ void tiled_body(const TiledArgs<TC> &a, int logical)
                         for (int v = 0; v < VW; v++) xr[g * VW + v] = t.v[v];
                     }
                 }
-                if (K::NEED_Y) {
+                if (K::NEED_Y && K::D1) {
+                    // staggered: A1*(u0 - u-1) + Aj*(u+(j-1) - u-j), j = 2..R (order 2: no weight)
+                    const V td = *reinterpret_cast<const V *>(pl + (ty + R - 1) * W + HL + tx * VW);
+#pragma unroll
+                    for (int v = 0; v < VW; v++) {
+                        const TS diff = cur.v[v] - td.v[v];
+                        ay[v] = R == 1 ? (TC)diff : a.c.D1[1] * (TC)diff;
+                    }
+#pragma unroll
+                    for (int j = 2; j <= R; j++) {
+                        const V tu = *reinterpret_cast<const V *>(pl + (ty + R + j - 1) * W + HL + tx * VW);
+                        const V tdj = *reinterpret_cast<const V *>(pl + (ty + R - j) * W + HL + tx * VW);
+#pragma unroll
+                        for (int v = 0; v < VW; v++) {
+                            const TS diff = tu.v[v] - tdj.v[v];
+                            ay[v] = ay[v] + a.c.D1[j] * (TC)diff;
+                        }
+                    }
+                } else if (K::NEED_Y) {
 #pragma unroll
                     for (int v = 0; v < VW; v++) ay[v] = A[0] * (TC)cur.v[v];
 #pragma unroll
@@ -504,22 +526,43 @@ __device__ __forceinline__ void tiled_body(const TiledArgs<TC> &a, int logical)
                     const TS c0 = cur.v[v];
                     TS p[R], m[R];
                     TC d1 = 0, d2 = 0, d3 = 0;
-                    if (K::NEED_X) {
+                    if (K::D1) {
+                        // staggered first derivative at i - 1/2: values at +(j-1) and -j
+                        if (K::NEED_X) {
 #pragma unroll
-                        for (int j = 1; j <= R; j++) {
-                            p[j - 1] = xr[HL + v + j];
-                            m[j - 1] = xr[HL + v - j];
+                            for (int j = 1; j <= R; j++) {
+                                p[j - 1] = xr[HL + v + j - 1];
+                                m[j - 1] = xr[HL + v - j];
+                            }
+                            d1 = d1_values<TS, TC, R>(p, m, a.c.D1, a.c.inv[0]);
                         }
-                        d1 = d2_values<TS, TC, R>(c0, p, m, A, a.c.inv2[0]);
-                    }
-                    if (K::NEED_Y) d2 = ay[v] * a.c.inv2[1];
-                    if (K::NEED_Z) {
+                        if (K::NEED_Y) d2 = ay[v] * a.c.inv[1];
+                        if (K::NEED_Z) {
 #pragma unroll
-                        for (int j = 1; j <= R; j++) {
-                            p[j - 1] = q[(s + R + j) % P].v[v];
-                            m[j - 1] = q[(s + R - j + P) % P].v[v];
+                            for (int j = 1; j <= R; j++) {
+                                p[j - 1] = q[(s + R + j - 1) % P].v[v];
+                                m[j - 1] = q[(s + R - j + P) % P].v[v];
+                            }
+                            d3 = d1_values<TS, TC, R>(p, m, a.c.D1, a.c.inv[2]);
                         }
-                        d3 = d2_values<TS, TC, R>(c0, p, m, A, a.c.inv2[2]);
+                    } else {
+                        if (K::NEED_X) {
+#pragma unroll
+                            for (int j = 1; j <= R; j++) {
+                                p[j - 1] = xr[HL + v + j];
+                                m[j - 1] = xr[HL + v - j];
+                            }
+                            d1 = d2_values<TS, TC, R>(c0, p, m, A, a.c.inv2[0]);
+                        }
+                        if (K::NEED_Y) d2 = ay[v] * a.c.inv2[1];
+                        if (K::NEED_Z) {
+#pragma unroll
+                            for (int j = 1; j <= R; j++) {
+                                p[j - 1] = q[(s + R + j) % P].v[v];
+                                m[j - 1] = q[(s + R - j + P) % P].v[v];
+                            }
+                            d3 = d2_values<TS, TC, R>(c0, p, m, A, a.c.inv2[2]);
+                        }
                     }
                     TS o;
                     if (MODE == MODE_AC2) {
@@ -527,9 +570,9 @@ __device__ __forceinline__ void tiled_body(const TiledArgs<TC> &a, int logical)
                         o = leapfrog<TS, TC>(c0, prn_v.v[v], coef_v.v[v], lap);
                     } else if (MODE == MODE_LAP) {
                         o = (TS)(d1 + d2 + d3);
-                    } else if (MODE == MODE_D2X) {
+                    } else if (MODE == MODE_D2X || MODE == MODE_D1X) {
                         o = (TS)d1;
-                    } else if (MODE == MODE_D2Y) {
+                    } else if (MODE == MODE_D2Y || MODE == MODE_D1Y) {
                         o = (TS)d2;
                     } else {
                         o = (TS)d3;

@@ -33,6 +33,24 @@ static const FD_Rational FD_D2_TABLE[8][9] = {
 // number of math ops of one D2 along one axis (src/fdm.h:880-921)
 static const Myint FD_D2_NOP[8] = {4, 8, 11, 14, 17, 20, 23, 26};
 
+// staggered first-derivative weights A1..AR for R = 1..8 (src/fdm.h:29-268; index 0 unused)
+static const FD_Rational FD_D1_TABLE[8][9] = {
+    {{0, 1}, {1, 1}},
+    {{0, 1}, {9, 8}, {-1, 24}},
+    {{0, 1}, {75, 64}, {-25, 384}, {3, 640}},
+    {{0, 1}, {1225, 1024}, {-245, 3072}, {49, 5120}, {-5, 7168}},
+    {{0, 1}, {19845, 16384}, {-735, 8192}, {567, 40960}, {-405, 229376}, {35, 294912}},
+    {{0, 1}, {160083, 131072}, {-12705, 131072}, {22869, 1310720}, {-5445, 1835008}, {847, 2359296},
+     {-63, 2883584}},
+    {{0, 1}, {1288287, 1048576}, {-429429, 4194304}, {429429, 20971520}, {-61347, 14680064},
+     {13013, 18874368}, {-3549, 46137344}, {231, 54525952}},
+    {{0, 1}, {41409225, 33554432}, {-3578575, 33554432}, {3864861, 167772160}, {-1254825, 234881024},
+     {325325, 301989888}, {-61425, 369098752}, {7425, 436207616}, {-143, 167772160}},
+};
+
+// number of math ops of one D1 along one axis (src/fdm.h:35-235, getFD_D1nMathOp :836-875)
+static const Myint FD_D1_NOP[8] = {2, 6, 9, 12, 15, 18, 21, 24};
+
 static inline bool validFdOrder(Myint fdOrder)
 {
     return fdOrder >= 2 && fdOrder <= 16 && fdOrder % 2 == 0;
@@ -76,6 +94,11 @@ static inline Myfloat64 getSumFD_D2Coef(Myint fdOrder)
 static inline Myint getFD_haloWidth(Myint fdOrder)
 {
     return validFdOrder(fdOrder) ? fdOrder / 2 : UNSPECIFIED;
+}
+
+static inline Myint getFD_D1nMathOp(Myint fdOrder)
+{
+    return validFdOrder(fdOrder) ? FD_D1_NOP[fdOrder / 2 - 1] : UNSPECIFIED;
 }
 
 static inline Myint getFD_D2nMathOp(Myint fdOrder)

@@ -366,12 +366,12 @@ void Grid::addArray(const Grid &, const Grid &) { notProvided("addArray"); }
 void Grid::multiplyArray(const Grid &, const Grid &) { notProvided("multiplyArray"); }
 void Grid::addUpdateArray(const Grid &) { notProvided("addUpdateArray"); }
 Rtn_code Grid::applyBoundaryCondition(BoundCond_type) { notProvided("applyBoundaryCondition"); return RTN_CODE_KO; }
-Rtn_code Grid::FD_D2_N1(Point_type, const Grid &, Myint) { notProvided("FD_D2_N1"); return RTN_CODE_KO; }
-Rtn_code Grid::FD_D2_N2(Point_type, const Grid &, Myint) { notProvided("FD_D2_N2"); return RTN_CODE_KO; }
-Rtn_code Grid::FD_D2_N3(Point_type, const Grid &, Myint) { notProvided("FD_D2_N3"); return RTN_CODE_KO; }
 Rtn_code Grid::FD_D1_N1(Point_type, const Grid &, Myint) { notProvided("FD_D1_N1"); return RTN_CODE_KO; }
 Rtn_code Grid::FD_D1_N2(Point_type, const Grid &, Myint) { notProvided("FD_D1_N2"); return RTN_CODE_KO; }
 Rtn_code Grid::FD_D1_N3(Point_type, const Grid &, Myint) { notProvided("FD_D1_N3"); return RTN_CODE_KO; }
+Rtn_code Grid::FD_D2_N1(Point_type, const Grid &, Myint) { notProvided("FD_D2_N1"); return RTN_CODE_KO; }
+Rtn_code Grid::FD_D2_N2(Point_type, const Grid &, Myint) { notProvided("FD_D2_N2"); return RTN_CODE_KO; }
+Rtn_code Grid::FD_D2_N3(Point_type, const Grid &, Myint) { notProvided("FD_D2_N3"); return RTN_CODE_KO; }
 Rtn_code Grid::FD_LAPLACIAN(Point_type, const Grid &, Myint) { notProvided("FD_LAPLACIAN"); return RTN_CODE_KO; }
 Rtn_code Grid::updatePressure(Point_type, const Grid &, const Grid &, const Grid &)
 {
@@ -567,7 +567,7 @@ Rtn_code Grid::myExchangeAll_halos()
     return getNumberOfNeighbour() == 0 ? RTN_CODE_OK : exchangeHalos(MPI_COMM_MODE_SENDRECV);
 }
 
-Myint Grid::getFlopPerPtFD_D1(Myint) { return UNSPECIFIED; }
+Myint Grid::getFlopPerPtFD_D1(Myint fdOrder) { return getFD_D1nMathOp(fdOrder); }
 Myint Grid::getFlopPerPtFD_D2(Myint fdOrder) { return getFD_D2nMathOp(fdOrder); }
 Myint Grid::getFlopPerPtFD_LAPLACIAN(Myint fdOrder)
 {

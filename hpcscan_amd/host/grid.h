@@ -84,12 +84,12 @@ public:
 
     // the stencil path (src/grid.cpp:3065-5112)
     virtual Rtn_code applyBoundaryCondition(BoundCond_type boundCondType);
-    virtual Rtn_code FD_D2_N1(Point_type pType, const Grid &Wgrid, Myint fdOrder);
-    virtual Rtn_code FD_D2_N2(Point_type pType, const Grid &Wgrid, Myint fdOrder);
-    virtual Rtn_code FD_D2_N3(Point_type pType, const Grid &Wgrid, Myint fdOrder);
     virtual Rtn_code FD_D1_N1(Point_type pType, const Grid &Wgrid, Myint fdOrder);
     virtual Rtn_code FD_D1_N2(Point_type pType, const Grid &Wgrid, Myint fdOrder);
     virtual Rtn_code FD_D1_N3(Point_type pType, const Grid &Wgrid, Myint fdOrder);
+    virtual Rtn_code FD_D2_N1(Point_type pType, const Grid &Wgrid, Myint fdOrder);
+    virtual Rtn_code FD_D2_N2(Point_type pType, const Grid &Wgrid, Myint fdOrder);
+    virtual Rtn_code FD_D2_N3(Point_type pType, const Grid &Wgrid, Myint fdOrder);
     virtual Rtn_code FD_LAPLACIAN(Point_type pType, const Grid &Wgrid, Myint fdOrder);
     virtual Rtn_code updatePressure(Point_type pType, const Grid &prcGrid, const Grid &coefGrid,
                                     const Grid &laplaGrid);

@@ -303,6 +303,30 @@ Rtn_code Grid_Mi355x::fdD2(int axis, const Grid &W, Myint fdOrder)
     return check(hsx_stream_sync(nullptr), "hsx_stream_sync") ? RTN_CODE_OK : RTN_CODE_KO;
 }
 
+Rtn_code Grid_Mi355x::fdD1(int axis, const Grid &W, Myint fdOrder)
+{
+    static const char *names[3] = {"Grid::FD_D1_N1, grids have not same size",
+                                   "Grid::FD_D1_N2, grids have not same size",
+                                   "Grid::FD_D1_N3, grids have not same size"};
+    if (!sameSize(W)) {
+        printError(names[axis]);
+        return RTN_CODE_KO;
+    }
+    if (fdOrder != 2 * haloWidth) {
+        printError(" In Grid_Mi355x, fdOrder does not match the grid halo width", fdOrder);
+        return RTN_CODE_KO;
+    }
+    const Grid_Mi355x *pw = peer(W, "FD_D1");
+    if (!pw || !settle()) return RTN_CODE_KO;
+    pw->touched();
+    hsx_grid g = desc();
+    if (!check(hsx_fd_d1(&g, axis, d_grid, pw->d_grid, nullptr), "hsx_fd_d1")) return RTN_CODE_KO;
+    return check(hsx_stream_sync(nullptr), "hsx_stream_sync") ? RTN_CODE_OK : RTN_CODE_KO;
+}
+
+Rtn_code Grid_Mi355x::FD_D1_N1(Point_type, const Grid &W, Myint o) { return fdD1(0, W, o); }
+Rtn_code Grid_Mi355x::FD_D1_N2(Point_type, const Grid &W, Myint o) { return fdD1(1, W, o); }
+Rtn_code Grid_Mi355x::FD_D1_N3(Point_type, const Grid &W, Myint o) { return fdD1(2, W, o); }
 Rtn_code Grid_Mi355x::FD_D2_N1(Point_type, const Grid &W, Myint o) { return fdD2(0, W, o); }
 Rtn_code Grid_Mi355x::FD_D2_N2(Point_type, const Grid &W, Myint o) { return fdD2(1, W, o); }
 Rtn_code Grid_Mi355x::FD_D2_N3(Point_type, const Grid &W, Myint o) { return fdD2(2, W, o); }

@@ -41,6 +41,9 @@ public:
     void addUpdateArray(const Grid &) override;
 
     Rtn_code applyBoundaryCondition(BoundCond_type) override;
+    Rtn_code FD_D1_N1(Point_type, const Grid &, Myint) override;
+    Rtn_code FD_D1_N2(Point_type, const Grid &, Myint) override;
+    Rtn_code FD_D1_N3(Point_type, const Grid &, Myint) override;
     Rtn_code FD_D2_N1(Point_type, const Grid &, Myint) override;
     Rtn_code FD_D2_N2(Point_type, const Grid &, Myint) override;
     Rtn_code FD_D2_N3(Point_type, const Grid &, Myint) override;
@@ -68,6 +71,7 @@ protected:
     const Grid_Mi355x *peer(const Grid &g, const char *what) const;
     Rtn_code toHost(void);
     Rtn_code fdD2(int axis, const Grid &W, Myint fdOrder);
+    Rtn_code fdD1(int axis, const Grid &W, Myint fdOrder);
 };
 
 } // namespace hpcscan

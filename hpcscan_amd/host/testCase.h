@@ -39,6 +39,13 @@ protected:
     shared_ptr<Hardware> hw;
 };
 
+class TestCase_FD_D1 : protected TestCase
+{
+public:
+    TestCase_FD_D1(void);
+    virtual Rtn_code run(void);
+};
+
 class TestCase_FD_D2 : protected TestCase
 {
 public:

@@ -142,6 +142,9 @@ int hsx_array_op(const hsx_grid *g, int op, void *w, const void *u, const void *
 int hsx_free_surface(const hsx_grid *g, void *u, int face_mask, void *stream);
 /* Grid::FD_D2_N1/N2/N3 (src/grid.cpp:3232-4065): w = D2 along axis (0,1,2) of u, inner points */
 int hsx_fd_d2(const hsx_grid *g, int axis, const void *u, void *w, void *stream);
+/* Grid::FD_D1_N1/N2/N3 (src/grid.cpp:3401-3560, 3735-3894, 4069-4229): w = staggered
+ * first derivative along axis (0,1,2) of u, i.e. at i - 1/2, inner points */
+int hsx_fd_d1(const hsx_grid *g, int axis, const void *u, void *w, void *stream);
 /* Grid::FD_LAPLACIAN (src/grid.cpp:4235-4577): w = Laplacian(u), inner points */
 int hsx_laplacian(const hsx_grid *g, const void *u, void *w, void *stream);
 /* Grid::updatePressure (src/grid.cpp:4579-4610): prn = 2 prc - prn + coef*lapla */

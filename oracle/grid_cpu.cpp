@@ -110,6 +110,9 @@ public:
     Rtn_code FD_D2_N1(Point_type, const Grid &W, Myint o) override { return fdD2(0, W, o); }
     Rtn_code FD_D2_N2(Point_type, const Grid &W, Myint o) override { return fdD2(1, W, o); }
     Rtn_code FD_D2_N3(Point_type, const Grid &W, Myint o) override { return fdD2(2, W, o); }
+    Rtn_code FD_D1_N1(Point_type, const Grid &W, Myint o) override { return fdD1(0, W, o); }
+    Rtn_code FD_D1_N2(Point_type, const Grid &W, Myint o) override { return fdD1(1, W, o); }
+    Rtn_code FD_D1_N3(Point_type, const Grid &W, Myint o) override { return fdD1(2, W, o); }
 
     Rtn_code FD_LAPLACIAN(Point_type, const Grid &W, Myint o) override
     {
@@ -256,6 +259,23 @@ private:
         g.n3 = n3;
         g.npoint = npoint;
         return g;
+    }
+
+    Rtn_code fdD1(int axis, const Grid &W, Myint o)
+    {
+        static const char *msg[3] = {"Grid::FD_D1_N1, grids have not same size",
+                                     "Grid::FD_D1_N2, grids have not same size",
+                                     "Grid::FD_D1_N3, grids have not same size"};
+        if (!sameSize(W) || o != 2 * haloWidth) {
+            printError(msg[axis]);
+            return RTN_CODE_KO;
+        }
+        orc_grid g = toOrc();
+        // const Myfloat2 inv_d1 = ONE / d1 (src/grid.cpp:3415-3417)
+        const Myfloat64 dd[3] = {d1, d2, d3};
+        Myfloat2 iv[3];
+        for (int a = 0; a < 3; a++) iv[a] = ONE / dd[a];
+        return ORC(fd_d1)(&g, axis, grid_3d, W.grid_3d, iv, sched()) == 0 ? RTN_CODE_OK : RTN_CODE_KO;
     }
 
     Rtn_code fdD2(int axis, const Grid &W, Myint o)

@@ -39,6 +39,30 @@ int orc_d2_coefs(int fd_order, double a[9])
     return 0;
 }
 
+int orc_d1_coefs(int fd_order, double a[9])
+{
+    /* staggered first-derivative weights A1..AR (src/fdm.h:29-268); a[0] unused */
+    for (int j = 0; j < 9; j++) a[j] = 0.0;
+    switch (fd_order) {
+    case 2:  a[1] = 1.0; break;
+    case 4:  a[1] = 9. / 8.; a[2] = -1. / 24.; break;
+    case 6:  a[1] = 75. / 64.; a[2] = -25. / 384.; a[3] = 3. / 640.; break;
+    case 8:  a[1] = 1225. / 1024.; a[2] = -245. / 3072.; a[3] = 49. / 5120.; a[4] = -5. / 7168.; break;
+    case 10: a[1] = 19845. / 16384.; a[2] = -735. / 8192.; a[3] = 567. / 40960.; a[4] = -405. / 229376.;
+             a[5] = 35. / 294912.; break;
+    case 12: a[1] = 160083. / 131072.; a[2] = -12705. / 131072.; a[3] = 22869. / 1310720.;
+             a[4] = -5445. / 1835008.; a[5] = 847. / 2359296.; a[6] = -63. / 2883584.; break;
+    case 14: a[1] = 1288287. / 1048576.; a[2] = -429429. / 4194304.; a[3] = 429429. / 20971520.;
+             a[4] = -61347. / 14680064.; a[5] = 13013. / 18874368.; a[6] = -3549. / 46137344.;
+             a[7] = 231. / 54525952.; break;
+    case 16: a[1] = 41409225. / 33554432.; a[2] = -3578575. / 33554432.; a[3] = 3864861. / 167772160.;
+             a[4] = -1254825. / 234881024.; a[5] = 325325. / 301989888.; a[6] = -61425. / 369098752.;
+             a[7] = 7425. / 436207616.; a[8] = -143. / 167772160.; break;
+    default: return -1;
+    }
+    return 0;
+}
+
 double orc_sum_abs_d2_coef(int fd_order)
 {
     /* getSumAbsFD_D2Coef (src/fdm.h:771-787): sum over the 2R+1 Myfloat2

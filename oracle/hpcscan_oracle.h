@@ -68,6 +68,8 @@ int orc_index_range(const orc_grid *g, int ptype, int64_t r[6]);
 
 /* FD D2 rational coefficients A0..AR in double (src/fdm.h:276-507). */
 int orc_d2_coefs(int fd_order, double a[9]);
+/* Staggered first-derivative weights A1..AR, exact rationals (src/fdm.h:29-268). */
+int orc_d1_coefs(int fd_order, double a[9]);
 /* Sum of |coef| over the full stencil (src/fdm.h:771-787). */
 double orc_sum_abs_d2_coef(int fd_order);
 /* CFL = sqrt(sum|O2| / (dim * sum|Ok|)) rounded to float (src/propagator_Ac2.cpp:410-439). */
@@ -95,6 +97,8 @@ int    orc_free_surface_##SFX(const orc_grid *g, TS *u);                        
 int    orc_free_surface_mask_##SFX(const orc_grid *g, TS *u, int face_mask);            \
 int    orc_fd_d2_##SFX(const orc_grid *g, int axis, const TS *u, TS *w,                 \
                        const TC inv2[3], orc_sched s);                                  \
+int    orc_fd_d1_##SFX(const orc_grid *g, int axis, const TS *u, TS *w,                 \
+                       const TC inv[3], orc_sched s);                                   \
 int    orc_laplacian_##SFX(const orc_grid *g, const TS *u, TS *w, const TC inv2[3],     \
                            orc_sched s);                                                \
 int    orc_update_pressure_##SFX(const orc_grid *g, TS *prn, const TS *prc,             \

@@ -82,6 +82,7 @@ def lib():
         L.orc_layout.argtypes = [P(OrcLayoutOpts), P(OrcGrid)]
         L.orc_index_range.argtypes = [P(OrcGrid), ctypes.c_int, P(ctypes.c_int64)]
         L.orc_d2_coefs.argtypes = [ctypes.c_int, P(ctypes.c_double)]
+        L.orc_d1_coefs.argtypes = [ctypes.c_int, P(ctypes.c_double)]
         L.orc_sum_abs_d2_coef.argtypes = [ctypes.c_int]
         L.orc_sum_abs_d2_coef.restype = ctypes.c_double
         L.orc_cfl.argtypes = [ctypes.c_int, ctypes.c_int]
@@ -102,6 +103,7 @@ def lib():
             getattr(L, f"orc_free_surface_{sfx}").argtypes = [P(OrcGrid), vp]
             getattr(L, f"orc_free_surface_mask_{sfx}").argtypes = [P(OrcGrid), vp, ctypes.c_int]
             getattr(L, f"orc_fd_d2_{sfx}").argtypes = [P(OrcGrid), ctypes.c_int, vp, vp, vp, OrcSched]
+            getattr(L, f"orc_fd_d1_{sfx}").argtypes = [P(OrcGrid), ctypes.c_int, vp, vp, vp, OrcSched]
             getattr(L, f"orc_laplacian_{sfx}").argtypes = [P(OrcGrid), vp, vp, vp, OrcSched]
             getattr(L, f"orc_update_pressure_{sfx}").argtypes = [P(OrcGrid), vp, vp, vp, vp]
             getattr(L, f"orc_pressure_fd_{sfx}").argtypes = [P(OrcGrid), vp, vp, vp, vp, OrcSched]
@@ -352,6 +354,27 @@ def free_surface(g, u, compute64=False, face_mask=0x3F):
         getattr(lib(), f"orc_free_surface_{_sfx(u.dtype, compute64)}")(ctypes.byref(g), _p(u))
     else:
         getattr(lib(), f"orc_free_surface_mask_{_sfx(u.dtype, compute64)}")(ctypes.byref(g), _p(u), face_mask)
+
+
+def inv1(d, dtype, compute64=False):
+    """inv_d = (Myfloat2)(ONE / d) of Grid::FD_D1_N* (src/grid.cpp:3415-3417)."""
+    t = np.float64 if (dtype == np.float64 or compute64) else np.float32
+    return np.array([t(1.0 / x) if x > 0 else np.inf for x in d], dtype=t)
+
+
+def d1_coefs(order):
+    """staggered D1 weights A1..AR (index 0 unused), as the oracle holds them"""
+    a = (ctypes.c_double * 9)()
+    if lib().orc_d1_coefs(order, a):
+        raise ValueError(order)
+    return list(a)
+
+
+def fd_d1(g, axis, u, w, iv, s=None, compute64=False):
+    """Grid::FD_D1_N1/N2/N3: staggered first derivative, w at i - 1/2 on inner points."""
+    s = s or sched()
+    return getattr(lib(), f"orc_fd_d1_{_sfx(u.dtype, compute64)}")(
+        ctypes.byref(g), axis, _p(u), _p(w), _p(iv), s)
 
 
 def fd_d2(g, axis, u, w, iv, s=None, compute64=False):

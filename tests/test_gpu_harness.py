@@ -53,6 +53,10 @@ def test_default_propa_matches_published(tmp_path):
     ["-testCase", "FD_D2", "-dim", "3", "-fdOrder", "16", "-n1", "27", "-n2", "36", "-n3", "41"],
     ["-testCase", "FD_D2", "-dim", "2", "-fdOrder", "2", "-n1AddPad", "5", "-n2AddPad", "5"],
     ["-testCase", "FD_D2", "-n1Offset", "1", "-n2Offset", "2", "-n1MulPad", "5"],
+    ["-testCase", "FD_D1", "-dim", "3", "-fdOrder", "8"],
+    ["-testCase", "FD_D1", "-dim", "3", "-fdOrder", "16", "-n1", "68", "-n2", "36", "-n3", "41"],
+    ["-testCase", "FD_D1", "-dim", "2", "-fdOrder", "2"],
+    ["-testCase", "FD_D1", "-dim", "1", "-fdOrder", "12"],
 ])
 def test_same_report_as_cpu_reference(tmp_path, prec, args):
     """Error values printed by the MI355X run equal those of the CPU run."""
@@ -101,7 +105,7 @@ def test_validation_matrix_zero_failed(tmp_path):
     """The stencil part of the reference's validation driver (script/testDriver.sh:79-141,
     233-321) on -testMode MI355X, one rank: 0 FAILED."""
     runs = []
-    for tc in ("FD_D2", "Propa"):
+    for tc in ("FD_D1", "FD_D2", "Propa"):
         for dim in ("1", "2", "3"):
             for order in ("2", "4", "6", "8", "10", "12", "14", "16"):
                 runs.append(["-testCase", tc, "-dim", dim, "-fdOrder", order])

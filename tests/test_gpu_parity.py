@@ -80,6 +80,11 @@ def test_stencils_bit_exact(oracle, capi, family, dim, n, order, prec):
         dw.upload(w0)
         capi.fd_d2(g, axis, du, dw)
         _same(dw.download(dtype), ref)
+        ref = w0.copy()
+        oracle.fd_d1(o, axis, u, ref, oracle.inv1(p.d, dtype, compute64=c64), compute64=c64)
+        dw.upload(w0)
+        capi.fd_d1(g, axis, du, dw)
+        _same(dw.download(dtype), ref)
 
     ref = w0.copy()
     oracle.laplacian(o, u, ref, iv, compute64=c64)
@@ -357,11 +362,17 @@ def test_tiled_shapes_bit_exact(oracle, capi, order, prec, n1, n2, n3, lanes, ch
     capi.set_kernel(capi.HSX_KERNEL_TILED)
     capi.set_tile(lanes, chunk, rows, streams)
     try:
+        iv1 = oracle.inv1(p.d, dtype, compute64=c64)
         for axis in range(3):
             ref = w0.copy()
             oracle.fd_d2(o, axis, u, ref, iv, compute64=c64)
             dw.upload(w0)
             capi.fd_d2(g, axis, du, dw)
+            _same(dw.download(dtype), ref)
+            ref = w0.copy()
+            oracle.fd_d1(o, axis, u, ref, iv1, compute64=c64)
+            dw.upload(w0)
+            capi.fd_d1(g, axis, du, dw)
             _same(dw.download(dtype), ref)
         ref = w0.copy()
         oracle.laplacian(o, u, ref, iv, compute64=c64)

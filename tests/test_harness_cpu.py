@@ -67,6 +67,23 @@ def test_cpu_grid_test_case(oracle, tmp_path, dim):
     assert len(line) == 4 + 9 + 18
 
 
+@pytest.mark.parametrize("dim", ["1", "2", "3"])
+@pytest.mark.parametrize("order", ["2", "8", "16"])
+def test_cpu_fd_d1_test_case(oracle, tmp_path, dim, order):
+    """-testCase FD_D1 (src/testCase_FD_D1.cpp): one staggered first derivative per
+    axis against the analytic cosine half a step back, L1 < 1e-3; 25 perf-log
+    columns after the common ones."""
+    rc, out = run_bin(CPU, ["-testCase", "FD_D1", "-testMode", "CacheBlk", "-ntry", "2", "-dim", dim,
+                            "-fdOrder", order], tmp_path)
+    assert rc == 0
+    cases = re.findall(r"\* Case\s+(\S+)", out)
+    assert cases == ["FD_D1Axis%d" % (k + 1) for k in range(int(dim))]
+    assert out.count("Test PASSED") == int(dim) and "FAILED" not in out
+    assert report_value(out, "#Point in stencil") == order
+    line = open(tmp_path / "hpcscan.perf.FD_D1.log").read().split()
+    assert len(line) == 4 + 9 + 25
+
+
 def test_split_and_standard_print_same_error(oracle, tmp_path):
     outs = []
     for prop in ("Ac2Standard", "Ac2SplitComp"):

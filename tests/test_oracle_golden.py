@@ -47,6 +47,78 @@ def test_fornberg_rejects_a_wrong_weight(oracle):
     assert sum(2 * q * j * j for j, q in enumerate(bad)) != 2
 
 
+def _staggered_d1_weights(R):
+    """Exact first-derivative weights at the 2R half-integer nodes +-(j - 1/2):
+    the unique antisymmetric weights that differentiate x, x^3, ..., x^(2R-1)
+    exactly (moment conditions, solved in rationals). Returns [None, A1..AR]
+    where A_j weighs u(+(j-1/2)) - u(-(j-1/2))."""
+    t = [Fraction(2 * j - 1, 2) for j in range(1, R + 1)]
+    # rows k = 0..R-1: sum_j A_j * 2 t_j^(2k+1) = (1 if k == 0 else 0)
+    M = [[2 * t[j] ** (2 * k + 1) for j in range(R)] + [Fraction(int(k == 0))] for k in range(R)]
+    for c in range(R):
+        piv = next(r for r in range(c, R) if M[r][c] != 0)
+        M[c], M[piv] = M[piv], M[c]
+        for r in range(R):
+            if r != c and M[r][c] != 0:
+                f = M[r][c] / M[c][c]
+                M[r] = [x - f * y for x, y in zip(M[r], M[c])]
+    return [None] + [M[j][R] / M[j][j] for j in range(R)]
+
+
+@pytest.mark.parametrize("order", [2, 4, 6, 8, 10, 12, 14, 16])
+def test_d1_weights_equal_exact_staggered_rationals(oracle, order):
+    """src/fdm.h:29-268: the staggered weights are the exact moment solution."""
+    R = order // 2
+    exact = _staggered_d1_weights(R)
+    got = oracle.d1_coefs(order)
+    assert got[0] == 0.0
+    assert [float(q) for q in exact[1:]] == got[1:R + 1]
+    assert sum(2 * exact[j] * Fraction(2 * j - 1, 2) for j in range(1, R + 1)) == 1
+
+
+def test_staggered_weights_reject_a_wrong_weight():
+    # a plausible slip (O4 A2 = -1/12, the centred value) breaks the cubic moment
+    w = _staggered_d1_weights(2)
+    assert w[1] == Fraction(9, 8) and w[2] == Fraction(-1, 24)
+    bad = [None, w[1], Fraction(-1, 12)]
+    assert sum(2 * bad[j] * Fraction(2 * j - 1, 2) ** 3 for j in (1, 2)) != 0
+
+
+@pytest.mark.parametrize("order", [2, 4, 8, 16])
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_oracle_d1_on_a_sine_equals_the_discrete_symbol(oracle, order, axis):
+    """D1 of sin(a x) along one axis is exactly S(a h) cos(a (x - h/2)) times the
+    other factors, S(theta) = (2/h) sum_j A_j sin((j - 1/2) theta): this fixes the
+    half-step shift, the stencil offsets and the signs (FP64 oracle, 1e-12)."""
+    R = order // 2
+    n = (23, 19, 17)
+    g = oracle.layout(3, list(n), order, storage_fp64=True)
+    d = [1.0 / (k - 1) for k in n]
+    A = oracle.d1_coefs(order)
+    a = (2.3, 1.7, 3.1)
+    coords = []
+    for ax in range(3):
+        s = g.ax[ax].inner_start
+        npts = (g.n1, g.n2, g.n3)[ax]
+        coords.append((np.arange(npts) - s) * d[ax])
+    x1, x2, x3 = np.meshgrid(coords[0], coords[1], coords[2], indexing="ij")
+    u = (np.sin(a[0] * x1) * np.sin(a[1] * x2) * np.sin(a[2] * x3)).transpose(2, 1, 0).ravel().copy()
+    w = np.zeros_like(u)
+    inv = np.array([1.0 / x for x in d])
+    oracle.fd_d1(g, axis, u, w, inv)
+    h = d[axis]
+    S = 2.0 / h * sum(A[j] * math.sin((j - 0.5) * a[axis] * h) for j in range(1, R + 1))
+    xs = [x1, x2, x3]
+    xs[axis] = xs[axis] - h / 2
+    fac = [np.sin(a[k] * xs[k]) if k != axis else np.cos(a[k] * xs[k]) for k in range(3)]
+    ref = (S * fac[0] * fac[1] * fac[2]).transpose(2, 1, 0).ravel()
+    r = oracle.index_range(g, oracle.INNER_POINTS)
+    sel = np.zeros((g.n3, g.n2, g.n1), bool)
+    sel[r[4]:r[5] + 1, r[2]:r[3] + 1, r[0]:r[1] + 1] = True
+    sel = sel.ravel()
+    assert np.max(np.abs(w[sel] - ref[sel])) < 1e-11 * np.max(np.abs(ref[sel]))
+
+
 def test_cfl_published(oracle):
     ref = json.load(open(os.path.join(GOLDEN, "propa_default_61.json")))
     assert g6(oracle.cfl(8, 3)) == ref["CFL"]
