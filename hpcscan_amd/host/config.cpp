@@ -139,7 +139,7 @@ std::vector<Flag> flagTable(Config &c)
         {"-ratioCFL", K::Float, &c.ratioCFL, " Ratio CFL\t", C::None, "ratio of stability dt for propagator"},
         {"-snapInc", K::Int, &c.snapInc, " Snapshots inc. (steps)", C::Pos, "snaphots increment (no. of time steps)"},
         {"-snapDt", K::Float, &c.snapDt, " Snapshots inc. (sec.)", C::Pos, "snaphots increment (time in sec.), overrides snapInc"},
-        {"-testCase", K::Str, &c.testCaseName, " TestCase name\t", C::None, "run specific testCase by name: FD_D1 | FD_D2 | Grid | Propa | All"},
+        {"-testCase", K::Str, &c.testCaseName, " TestCase name\t", C::None, "run specific testCase by name: FD_D1 | FD_D2 | Grid | Modeling | Propa | All"},
         {"-testMode", K::Str, &c.testMode, " Test mode\t", C::None, "test mode (see enabled modes in the header)"},
         {"-tmax", K::Float, &c.tmax, " Tmax\t\t", C::Pos, "max. time (s) for propagator, overrides nt"},
         {"-writeGrid", K::On, &c.writeGrid, " Write grids on disk", C::None, "write grids on disk"},

@@ -321,23 +321,94 @@ Rtn_code Grid::write(Point_type pointType, string fileName)
     return RTN_CODE_OK;
 }
 
-Rtn_code Grid::writeGlobal(Point_type, string, Myint)
+// snapshot num_iter of the inner points appended to <file>.global.grid.bin, one
+// rank (src/grid.cpp:1358-1441: the MPI-IO view of the global inner grid)
+Rtn_code Grid::writeGlobal(Point_type pointType, string fileName, Myint num_iter)
 {
-    // MPI-IO global snapshots are not part of this single-process build
-    printError(" In Grid::writeGlobal, not available in this build");
-    return RTN_CODE_KO;
+    if (!Config::Instance()->writeGrid) {
+        grid_writeGlobal_time = 0.0;
+        return RTN_CODE_OK;
+    }
+    const double t0 = hpcWtime();
+    const string fbin = fileName + ".global.grid.bin";
+    printInfo(MASTER, "* write on disk\t", fbin.c_str());
+    Myint64 i1s, i1e, i2s, i2e, i3s, i3e;
+    getGridIndex(pointType, &i1s, &i1e, &i2s, &i2e, &i3s, &i3e);
+    const Myint64 nRow = i1e - i1s + 1;
+    const Myint64 snap = nRow * (i2e - i2s + 1) * (i3e - i3s + 1);
+    fstream out(fbin.c_str(), ios::binary | ios::in | ios::out);
+    if (!out.is_open()) out.open(fbin.c_str(), ios::binary | ios::out);
+    if (!out.is_open()) {
+        printError(" In Grid::writeGlobal, cannot open", fbin);
+        return RTN_CODE_KO;
+    }
+    out.seekp((std::streamoff)(sizeof(Myfloat) * snap * num_iter), ios::beg);
+    for (Myint64 i3 = i3s; i3 <= i3e; i3++)
+        for (Myint64 i2 = i2s; i2 <= i2e; i2++)
+            out.write((const char *)&grid_3d[i1s + i2 * n1 + i3 * n2 * n1], nRow * sizeof(Myfloat));
+    out.close();
+    ofstream info(fileName + ".global.grid.info");
+    info << Config::Instance()->n1 << "\n" << Config::Instance()->n2 << "\n" << Config::Instance()->n3 << "\n";
+    grid_writeGlobal_time = hpcWtime() - t0;
+    printInfo(MASTER, "* time to write (s)", grid_writeGlobal_time);
+    return RTN_CODE_OK;
 }
 
-Rtn_code Grid::read(string)
+// the whole array, as Grid::write stores it (src/grid.cpp:1560-1583)
+Rtn_code Grid::read(string fileName)
 {
-    printError(" In Grid::read, not available in this build");
-    return RTN_CODE_KO;
+    const double t0 = hpcWtime();
+    printInfo(MASTER, "* read on disk\t", fileName.c_str());
+    ifstream in(fileName.c_str(), ios::binary);
+    if (!in.is_open()) {
+        printError(" In Grid::read, cannot open", fileName);
+        return RTN_CODE_KO;
+    }
+    in.read((char *)grid_3d, npoint * sizeof(Myfloat));
+    if (!in) {
+        printError(" In Grid::read, file too short", fileName);
+        return RTN_CODE_KO;
+    }
+    const double t = hpcWtime() - t0;
+    i_o_pread_time = t;
+    printInfo(MASTER, "* read time (s)\t", t);
+    printInfo(MASTER, "* read bwth (GB/s)", npoint * sizeof(Myfloat) / t / 1e9);
+    return RTN_CODE_OK;
 }
 
-Rtn_code Grid::read(Point_type, string)
+// the points of pointType from a file holding the global inner grid (n1 x n2 x n3
+// of the command line, n1 fastest), row by row (src/grid.cpp:1587-1643)
+Rtn_code Grid::read(Point_type pointType, string fileName)
 {
-    printError(" In Grid::read, not available in this build");
-    return RTN_CODE_KO;
+    if (pointType == ALL_POINTS) return this->read(fileName);
+    const double t0 = hpcWtime();
+    printInfo(MASTER, "* read on disk\t", fileName.c_str());
+    ifstream in(fileName.c_str(), ios::binary);
+    if (!in.is_open()) {
+        printError(" In Grid::read, cannot open", fileName);
+        return RTN_CODE_KO;
+    }
+    Myint64 i1s, i1e, i2s, i2e, i3s, i3e;
+    getGridIndex(pointType, &i1s, &i1e, &i2s, &i2e, &i3s, &i3e);
+    const Myint64 nRead = i1e - i1s + 1;
+    const Myint64 g1 = Config::Instance()->n1, g2 = Config::Instance()->n2;
+    for (Myint64 i3 = i3s; i3 <= i3e; i3++) {
+        for (Myint64 i2 = i2s; i2 <= i2e; i2++) {
+            const Myint64 off = (Myint64)sizeof(Myfloat) * (i1OffsetGlobInner + g1 * (i2OffsetGlobInner + i2 - i2s) +
+                                                             g1 * g2 * (i3OffsetGlobInner + i3 - i3s));
+            in.seekg((std::streamoff)off, ios::beg);
+            in.read((char *)&grid_3d[i1s + i2 * n1 + i3 * n2 * n1], nRead * sizeof(Myfloat));
+            if (!in) {
+                printError(" In Grid::read, file too short", fileName);
+                return RTN_CODE_KO;
+            }
+        }
+    }
+    const double t = hpcWtime() - t0;
+    i_o_pread_time = t;
+    printInfo(MASTER, "* read time (s)\t", t);
+    printInfo(MASTER, "* read bwth (GB/s)", npoint * sizeof(Myfloat) / t / 1e9);
+    return RTN_CODE_OK;
 }
 
 void Grid::notProvided(const char *method) const

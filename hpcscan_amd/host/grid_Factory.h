@@ -19,12 +19,15 @@
 
 namespace hpcscan {
 
+class DataAcquisition;
+
 struct GridBackend {
     string mode;
     std::function<Grid *(Grid_type)> create;
     std::function<Grid *(Grid_type, Dim_type, Myint64, Myint64, Myint64)> createSized;
     std::function<Hardware *(string)> hardware;
     std::function<void(void)> shutdown;   // release process-wide resources, may be empty
+    std::function<DataAcquisition *(void)> acquisition;   // may be empty: host acquisition
 };
 
 class Grid_Factory

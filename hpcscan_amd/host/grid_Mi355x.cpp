@@ -161,6 +161,31 @@ Rtn_code Grid_Mi355x::write(Point_type p, string fileName)
     return Grid::write(p, fileName);
 }
 
+Rtn_code Grid_Mi355x::writeGlobal(Point_type p, string fileName, Myint n_iter)
+{
+    if (!Config::Instance()->writeGrid) {
+        grid_writeGlobal_time = 0.0;
+        return RTN_CODE_OK;
+    }
+    if (toHost() != RTN_CODE_OK) return RTN_CODE_KO;
+    return Grid::writeGlobal(p, fileName, n_iter);
+}
+
+// the file is read into the host copy over the current device values, then the
+// whole grid goes back to the device
+Rtn_code Grid_Mi355x::read(string fileName) { return read(ALL_POINTS, fileName); }
+
+Rtn_code Grid_Mi355x::read(Point_type p, string fileName)
+{
+    if (toHost() != RTN_CODE_OK) return RTN_CODE_KO;
+    const Rtn_code rc = p == ALL_POINTS ? Grid::read(fileName) : Grid::read(p, fileName);
+    if (rc != RTN_CODE_OK) return rc;
+    touched();
+    if (!check(hsx_memcpy_h2d(d_grid, grid_3d, npoint * sizeof(Myfloat), nullptr), "hsx_memcpy_h2d"))
+        return RTN_CODE_KO;
+    return check(hsx_stream_sync(nullptr), "hsx_stream_sync") ? RTN_CODE_OK : RTN_CODE_KO;
+}
+
 // ---- fills ------------------------------------------------------------------
 void Grid_Mi355x::fill(Point_type p, Myfloat val)
 {
@@ -493,11 +518,14 @@ Myfloat Grid_Mi355x::getMax(Point_type p)
 }
 
 // ---- registration: -testMode MI355X ------------------------------------------------
+DataAcquisition *newDataAcquisition_Mi355x(void);   // data_Acquisition_Mi355x.cpp
+
 static bool registered = Grid_Factory::registerBackend(
     {GRID_MODE_MI355X, [](Grid_type t) -> Grid * { return new Grid_Mi355x(t); },
      [](Grid_type t, Dim_type d, Myint64 a, Myint64 b, Myint64 c) -> Grid * {
          return new Grid_Mi355x(t, d, a, b, c);
      },
-     [](string mode) -> Hardware * { return new Hardware_Mi355x(mode); }, nullptr});
+     [](string mode) -> Hardware * { return new Hardware_Mi355x(mode); }, nullptr,
+     [] { return newDataAcquisition_Mi355x(); }});
 
 } // namespace hpcscan

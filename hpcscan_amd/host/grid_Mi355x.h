@@ -24,6 +24,9 @@ public:
     void info(void) override;
     Rtn_code write(string) override;
     Rtn_code write(Point_type, string) override;
+    Rtn_code writeGlobal(Point_type, string, Myint) override;
+    Rtn_code read(string) override;
+    Rtn_code read(Point_type, string) override;
 
     Myfloat L1Err(Point_type, const Grid &) const override;
     Myfloat maxErr(Point_type, const Grid &) const override;
@@ -53,6 +56,11 @@ public:
 
     // device copy of the grid values (npoint * sizeof(Myfloat) bytes)
     void *d_grid;
+    // C-ABI descriptor of this grid
+    hsx_grid desc(void) const;
+    // values changed behind the backend's back (source injection): any halo
+    // values held for the folded free surface are stale
+    void touched(void) const { fsHalo = 0; }
 
 protected:
     // Free surface folded into the next step.  applyBoundaryCondition on a grid
@@ -64,8 +72,6 @@ protected:
     mutable int fsPending = 0;
     mutable int fsHalo = 0;
     bool settle(void) const;
-    void touched(void) const { fsHalo = 0; }
-    hsx_grid desc(void) const;
     void range(Point_type, int64_t r[6]) const;
     bool check(int rc, const char *what) const;
     const Grid_Mi355x *peer(const Grid &g, const char *what) const;

@@ -1,5 +1,5 @@
 // hpcscan host driver for the stencil path (reference behaviour: src/main.cpp:32-159).
-// Runs the FD_D1, FD_D2, Grid and Propa test cases selected with -testCase on the grid mode
+// Runs the FD_D1, FD_D2, Grid, Modeling and Propa test cases selected with -testCase on the grid mode
 // selected with -testMode.  One process, no MPI.
 #include "config.h"
 #include "global.h"
@@ -35,6 +35,10 @@ int main(int argc, char *argv[])
     }
     {
         TestCase_Grid t;
+        if (t.run() == RTN_CODE_OK) numTestCase++;
+    }
+    {
+        TestCase_Modeling t;
         if (t.run() == RTN_CODE_OK) numTestCase++;
     }
     {

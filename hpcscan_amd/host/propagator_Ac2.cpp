@@ -57,10 +57,21 @@ Rtn_code Propagator_Ac2::initialize(PropaInit_type propaInitType)
         prcGrid->defineUnitGrid();
         const Myfloat Vp = 1.0;
         minVelocity = maxVelocity = Vp;
+    } else if (propaInitType == MODELING) {
+        // Vp model read from disk into coefGrid (src/propagator_Ac2.cpp:138-160)
+        if (cfg->modelVpFile == "UNSPECIFIED") {
+            printError("In Propagator_Ac2::initialize, modelVpFile has not been specified");
+            return RTN_CODE_KO;
+        }
+        coefGrid->fill(ALL_POINTS, 0.0);
+        const double t0 = hpcWtime();
+        if (coefGrid->read(INNER_POINTS, cfg->modelVpFile) != RTN_CODE_OK) return RTN_CODE_KO;
+        coef_read_total_time = hpcWtime() - t0;
+        i_o_read_time = coefGrid->i_o_pread_time;
+        minVelocity = coefGrid->allProcGetMin(INNER_POINTS);
+        maxVelocity = coefGrid->allProcGetMax(INNER_POINTS);
     } else {
-        // velocity models read from disk belong to the Modeling test case
-        printError("In Propagator_Ac2::initialize, propaInitType not supported in this build",
-                   (Myint)propaInitType);
+        printError("In Propagator_Ac2::initialize, propaInitType not supported", (Myint)propaInitType);
         return RTN_CODE_KO;
     }
     if (minVelocity <= 0.0) {
@@ -94,20 +105,38 @@ Rtn_code Propagator_Ac2::initialize(PropaInit_type propaInitType)
         tmax = (nt - 1) * dt;
     }
 
-    coefGrid->fill(INNER_POINTS, minVelocity * minVelocity * dt * dt);
+    if (propaInitType == EIGEN_MODE) {
+        coefGrid->fill(INNER_POINTS, minVelocity * minVelocity * dt * dt);
+    } else {
+        // coef = vp * vp * dt * dt, left to right in Myfloat (src/propagator_Ac2.cpp:236-258),
+        // as grid operations so that it runs where the grid lives (the halos stay 0)
+        auto dtGrid = Grid_Factory::create(gridMode, GRID_LOCAL);
+        if (!dtGrid || dtGrid->initializeGrid() != RTN_CODE_OK) return RTN_CODE_KO;
+        dtGrid->fillArray(dt);
+        coefGrid->multiplyArray(*coefGrid, *coefGrid);
+        coefGrid->multiplyArray(*coefGrid, *dtGrid);
+        coefGrid->multiplyArray(*coefGrid, *dtGrid);
+    }
     coefGrid->write("coefGrid");
     i_o_write_coefgrid_time = coefGrid->grid_write_time;
 
-    // prn at t = -2 dt, prc at t = -dt (src/propagator_Ac2.cpp:270-285)
-    Myfloat timeSec = -2.0 * dt;
-    if (initializeGrid(*prnGrid, propaInitType, timeSec) != RTN_CODE_OK) {
-        printError("In Propagator_Ac2::initialize, initializeGrid not Ok");
-        return RTN_CODE_KO;
-    }
-    timeSec = -dt;
-    if (initializeGrid(*prcGrid, propaInitType, timeSec) != RTN_CODE_OK) {
-        printError("In Propagator_Ac2::initialize, initializeGrid not Ok");
-        return RTN_CODE_KO;
+    if (propaInitType == EIGEN_MODE) {
+        // prn at t = -2 dt, prc at t = -dt (src/propagator_Ac2.cpp:270-285)
+        Myfloat timeSec = -2.0 * dt;
+        if (initializeGrid(*prnGrid, propaInitType, timeSec) != RTN_CODE_OK) {
+            printError("In Propagator_Ac2::initialize, initializeGrid not Ok");
+            return RTN_CODE_KO;
+        }
+        timeSec = -dt;
+        if (initializeGrid(*prcGrid, propaInitType, timeSec) != RTN_CODE_OK) {
+            printError("In Propagator_Ac2::initialize, initializeGrid not Ok");
+            return RTN_CODE_KO;
+        }
+    } else {
+        // zero wavefields, maximum frequency from the command line (src/propagator_Ac2.cpp:287-296)
+        prnGrid->fill(ALL_POINTS, 0.0);
+        prcGrid->fill(ALL_POINTS, 0.0);
+        freqMax = cfg->freqMax;
     }
 
     halo_comm_time = compute_pressure_time = boundary_condition_time = 0.0;

@@ -39,6 +39,13 @@ protected:
     shared_ptr<Hardware> hw;
 };
 
+class TestCase_Modeling : protected TestCase
+{
+public:
+    TestCase_Modeling(void);
+    virtual Rtn_code run(void);
+};
+
 class TestCase_FD_D1 : protected TestCase
 {
 public:

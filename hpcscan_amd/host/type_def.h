@@ -65,6 +65,8 @@ enum PropaInit_type {EIGEN_MODE, MODELING};
 
 enum Modeling_type {FORWARD, ADJOINT};
 
+enum DataAcqui_type {ACQUI_BUILTIN};
+
 // hardware counter sample (src/type_def.h:46-55)
 typedef struct {
     double timeOfMeasure;

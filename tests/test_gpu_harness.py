@@ -89,6 +89,41 @@ def test_folded_free_surface_same_report(tmp_path, prec, args):
 
 
 @pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("dim,n,extra", [
+    ("3", (40, 50, 60), []),
+    ("3", (72, 40, 36), ["-fdOrder", "16"]),
+    ("3", (72, 40, 36), ["-hipFoldBC"]),
+    ("3", (64, 48, 40), ["-propagator", "Ac2SplitComp"]),
+    ("2", (40, 50), []),
+    ("1", (60,), []),
+])
+def test_modeling_traces_equal_cpu(tmp_path, prec, dim, n, extra):
+    """-testCase Modeling on the MI355X backend (device source term, receivers
+    and traces): the trace file equals the CPU run's byte for byte."""
+    from test_harness_cpu import MODELING_ARGS, write_vp_model
+
+    print("modeling", dim, n, extra, prec, flush=True)
+    write_vp_model(tmp_path / "vp.bin", n, fp64=prec == "fp64")
+    size = []
+    for k, a in zip(("-n1", "-n2", "-n3"), n):
+        size += [k, str(a)]
+    args = MODELING_ARGS + ["-dim", dim, "-modelVpFile", str(tmp_path / "vp.bin")] + size
+    gdir, cdir = tmp_path / "gpu", tmp_path / "cpu"
+    gdir.mkdir()
+    cdir.mkdir()
+    gpu_extra = [x for x in extra if x == "-hipFoldBC"]
+    both = [x for x in extra if x != "-hipFoldBC"]
+    rc1, gpu = run_bin(binary(prec), args + both + gpu_extra + ["-testMode", "MI355X"], gdir)
+    rc2, cpu = run_bin(CPU_BIN[prec], args + both + ["-testMode", "CacheBlk"], cdir)
+    assert rc1 == 0 and rc2 == 0, gpu[-2000:]
+    g = open(gdir / "ModelingForwardPrn.trace.bin", "rb").read()
+    c = open(cdir / "ModelingForwardPrn.trace.bin", "rb").read()
+    assert len(g) == len(c) > 0
+    assert g == c
+    assert "F A T A L" not in gpu
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
 @pytest.mark.parametrize("dim", ["1", "2", "3"])
 def test_grid_test_case_on_gpu(tmp_path, prec, dim):
     """-testCase Grid on the MI355X backend: the nine operations pass their

@@ -84,6 +84,49 @@ def test_cpu_fd_d1_test_case(oracle, tmp_path, dim, order):
     assert len(line) == 4 + 9 + 25
 
 
+def write_vp_model(path, n, vmin=1500.0, grad=40.0, fp64=False):
+    """velocity increasing with depth (n1 first) in the storage precision (Myfloat),
+    the inner grid of the command line with n1 fastest, as Grid::read expects
+    (src/grid.cpp:1587-1643)"""
+    import numpy as np
+
+    t = np.float64 if fp64 else np.float32
+    n = list(n) + [1] * (3 - len(n))
+    v = (vmin + grad * np.arange(n[0])).astype(t)
+    np.broadcast_to(v[None, None, :], (n[2], n[1], n[0])).astype(t).tofile(path)
+
+
+MODELING_ARGS = ["-testCase", "Modeling", "-fdOrder", "8", "-d1", "10", "-d2", "10", "-d3", "10", "-freqMax", "20",
+                 "-nt", "120", "-ntry", "1"]
+
+
+@pytest.mark.parametrize("dim,n,nrec", [("1", (60,), 1), ("2", (40, 50), 50), ("3", (40, 50, 60), 300)])
+def test_cpu_modeling_test_case(oracle, tmp_path, dim, n, nrec):
+    """-testCase Modeling (src/testCase_Modeling.cpp): Vp model read from disk,
+    Ricker source set one point below the top, receivers on that plane; the
+    trace file holds nt samples per receiver, the .info file nt, nRec, 1."""
+    import numpy as np
+
+    write_vp_model(tmp_path / "vp.bin", n)
+    size = []
+    for k, a in zip(("-n1", "-n2", "-n3"), n):
+        size += [k, str(a)]
+    rc, out = run_bin(CPU, MODELING_ARGS + ["-testMode", "CacheBlk", "-dim", dim, "-modelVpFile",
+                                            str(tmp_path / "vp.bin")] + size, tmp_path)
+    assert rc == 0, out[-2000:]
+    assert report_value(out, "Number of receivers") == str(nrec)
+    assert open(tmp_path / "ModelingForwardPrn.trace.info").read().split() == ["120", str(nrec), "1"]
+    t = np.fromfile(tmp_path / "ModelingForwardPrn.trace.bin", np.float32).reshape(nrec, 120)
+    assert np.isfinite(t).all() and np.abs(t).max() > 0.1
+    line = open(tmp_path / "hpcscan.perf.Modeling.log").read().split()
+    assert len(line) == 4 + 9 + 22
+
+
+def test_cpu_modeling_requires_a_model(oracle, tmp_path):
+    rc, out = run_bin(CPU, ["-testCase", "Modeling", "-testMode", "CacheBlk", "-nt", "5"], tmp_path)
+    assert "modelVpFile has not been specified" in out
+
+
 def test_split_and_standard_print_same_error(oracle, tmp_path):
     outs = []
     for prop in ("Ac2Standard", "Ac2SplitComp"):
