@@ -260,10 +260,18 @@ def adjoint_leg(args):
     inner = np.zeros((p.n3, p.n2, p.n1), bool)
     inner[r[4]:r[5] + 1, r[2]:r[3] + 1, r[0]:r[1] + 1] = True
     inner = inner.ravel()
-    src = np.array([acq.source_top_center(p, 1)], np.int64)
+    nshots = max(1, args.adj_shots)
+    if nshots == 1:
+        srcs = [acq.source_top_center(p, 1)]
+    else:   # sources spread along n2 one point below the top, centre of n3
+        N2 = r[3] - r[2] + 1
+        srcs = [acq.flat_index(p, r[0] + 1, r[2] + (k + 1) * N2 // (nshots + 1), (r[4] + r[5]) // 2)
+                for k in range(nshots)]
+    src = np.array(srcs[:1], np.int64)
     rec = acq.receiver_line(p, 200, 1)
     wav = acq.ricker(nt, dt, f0)
-    log(f"adjoint: {n}^3 O{order}, nt {nt}, dt {dt:.3e} s, {len(rec)} receivers, checkpoints every {K} ({args.adj_format})")
+    log(f"adjoint: {n}^3 O{order}, nt {nt}, dt {dt:.3e} s, {len(rec)} receivers, checkpoints every {K} "
+        f"({args.adj_format}), {nshots} shot(s)")
 
     def dev(a):
         return capi.DeviceArray.from_numpy(np.ascontiguousarray(a))
@@ -273,16 +281,24 @@ def adjoint_leg(args):
         g = capi.hsx_grid(lay[fp64])
         coef = dev(np.where(inner, dt * dt / m, 0.0).astype(dtp))
         coef_t = dev(np.where(inner, dt * dt / m_true, 0.0).astype(dtp))
-        ds, dr, dw = dev(src), dev(rec), dev(wav.astype(dtp).reshape(nt, 1))
-        shot = capi.make_shot(nt, dt, (ds, 1), dw, (dr, len(rec)))
-        return dict(g=g, coef=coef, coef_t=coef_t, keep=(ds, dr, dw), shot=shot, dtype=dtp)
+        return dict(g=g, coef=coef, coef_t=coef_t, dtype=dtp)
+
+    def with_source(S, s_idx):
+        ds, dr, dw = dev(np.array([s_idx], np.int64)), dev(rec), dev(wav.astype(S["dtype"]).reshape(nt, 1))
+        S["keep"] = (ds, dr, dw)
+        S["shot"] = capi.make_shot(nt, dt, (ds, 1), dw, (dr, len(rec)))
+
+    def observed(S64):
+        # observed data: FP64 forward of the perturbed model (FP32 runs use it rounded)
+        d64 = capi.DeviceArray(nt * len(rec) * 8)
+        capi.forward(S64["g"], S64["coef_t"], S64["shot"], d64)
+        dobs64 = d64.download(np.float64).reshape(nt, len(rec))
+        return d64, dobs64, dev(dobs64.astype(np.float32))
 
     S64, S32 = setup(True), setup(False)
-    # observed data: FP64 forward of the perturbed model (FP32 runs use it rounded)
-    d64 = capi.DeviceArray(nt * len(rec) * 8)
-    capi.forward(S64["g"], S64["coef_t"], S64["shot"], d64)
-    dobs64 = d64.download(np.float64).reshape(nt, len(rec))
-    dobs32 = dev(dobs64.astype(np.float32))
+    with_source(S64, srcs[0])
+    with_source(S32, srcs[0])
+    d64, dobs64, dobs32 = observed(S64)
 
     def grad_of(S, dobs, every, f, reps=1):
         best, gout, J = 1e30, None, None
@@ -297,11 +313,25 @@ def adjoint_leg(args):
         return best, gout, J
 
     t64, g64, J64 = grad_of(S64, d64, args.adj_fp64_every or K, capi.HSX_CKPT_FP32)
-    ours_t, g_ours, J32 = grad_of(S32, dobs32, K, fmt, reps=3)
+    ours_t, g_ours, J32 = grad_of(S32, dobs32, K, fmt, reps=3 if nshots == 1 else 1)
+    # further shots: gradients and misfits summed over the shots (FP64 on the host)
+    for s_idx in srcs[1:]:
+        with_source(S64, s_idx)
+        with_source(S32, s_idx)
+        d64, _, dobs32 = observed(S64)
+        t, g, J = grad_of(S64, d64, args.adj_fp64_every or K, capi.HSX_CKPT_FP32)
+        t64, g64, J64 = t64 + t, g64 + g, J64 + J
+        t, g, J = grad_of(S32, dobs32, K, fmt)
+        ours_t, g_ours, J32 = ours_t + t, g_ours + g, J32 + J
+        log(f"adjoint: shot at {s_idx} done ({t:.3f} s)")
+    if nshots > 1:
+        log(f"adjoint: {nshots} shots, {ours_t:.3f} s in all ({ours_t / nshots:.3f} s per shot), FP64 {t64:.3f} s")
+        ours_t /= nshots
+        t64 /= nshots
     nrm = np.linalg.norm(g64)
     err_ours = float(np.linalg.norm(g_ours - g64) / nrm)
     full_t = err_full = None
-    if (nt + 2) * p.npoint * 4 < 150e9:      # every step kept: only where it fits in HBM
+    if nshots == 1 and (nt + 2) * p.npoint * 4 < 150e9:      # every step kept: only where it fits in HBM
         full_t, g_full, _ = grad_of(S32, dobs32, 0, capi.HSX_CKPT_FP32, reps=2)
         err_full = float(np.linalg.norm(g_full - g64) / nrm)
     full_txt = "skipped" if full_t is None else f"{full_t:.3f} s (err {err_full:.2e})"
@@ -309,7 +339,7 @@ def adjoint_leg(args):
 
     # plain PyTorch, FP32, autograd over the whole time loop
     plain_t, err_plain = None, None
-    if not args.no_plain:
+    if not args.no_plain and nshots == 1:
         mt = torch.tensor(m, dtype=torch.float32, device="cuda", requires_grad=True)
         dobs_t = torch.tensor(dobs64.astype(np.float32), device="cuda")
         torch.cuda.synchronize()
@@ -325,7 +355,9 @@ def adjoint_leg(args):
         torch.cuda.empty_cache()
         log(f"adjoint: plain PyTorch {plain_t:.3f} s (err {err_plain:.2e})")
     return {
-        "workload": f"one shot misfit gradient, {n}^3 FP32 FD order {order}, {nt} steps, h 10 m, smooth velocity "
+        "workload": (f"one shot misfit gradient" if nshots == 1 else f"misfit gradient summed over {nshots} shots "
+                     f"(sources along n2), one after the other") +
+                    f", {n}^3 FP32 FD order {order}, {nt} steps, h 10 m, smooth velocity "
                     f"2000-4000 m/s, 8 Hz Ricker top centre, {len(rec)} receivers, data from a +-3% perturbed model",
         "s_per_shot": round(ours_t, 4),
         "checkpoints": f"every {K} steps, {args.adj_format} with per-snapshot max-abs scale, FP64 gradient",
@@ -444,6 +476,9 @@ def main():
     ap.add_argument("--adj-fp64-every", type=int, default=0,
                     help="checkpoint interval of the FP64 reference gradient (0 = the same as --adj-every)")
     ap.add_argument("--adj-every", type=int, default=20)
+    ap.add_argument("--adj-shots", type=int, default=1,
+                    help="shots whose gradients are summed (one after the other: 8 shots of 512^3 x 2000 steps "
+                         "need ~0.6 TB of checkpoints together)")
     ap.add_argument("--adj-format", default="fp16", choices=["fp16", "bf16", "fp32"])
     args = ap.parse_args()
 
