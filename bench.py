@@ -159,6 +159,22 @@ def gpu_leg(args):
     if not math.isfinite(s) or s <= 0:
         raise RuntimeError(f"wavefield is not finite after the run (sum|p| = {s})")
 
+    # pointwise reference on the same grid: updatePressure (prn = 2 prc - prn +
+    # coef * lapla, 4 reads + 1 write, no stencil, one-shot row launch) -- what
+    # plain streaming of this layout reaches on this device
+    pw_ms = None
+    if args.pointwise_ref:
+        lapla = capi.DeviceArray(nbytes)
+        capi.array_op(g, capi.HSX_ARRAY_COPY, lapla, coef)
+        pev = [(capi.Event(), capi.Event()) for _ in range(6)]
+        for e0, e1 in pev:
+            e0.record()
+            capi.update_pressure(g, a, b, coef, lapla)
+            e1.record()
+        capi.sync()
+        pw_ms = sorted(e0.elapsed_ms(e1) for e0, e1 in pev[1:])[2]
+        lapla.free()
+
     avg_kernel_s = sum(kern_ms) / len(kern_ms) / 1e3
     alg_bytes = 16 * npt
     achieved = alg_bytes / avg_kernel_s / 1e9
@@ -179,6 +195,9 @@ def gpu_leg(args):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "npt": npt, "device": info, "dt": float(dt),
+        "pointwise_ref": None if pw_ms is None else {
+            "kernel": "k_update_pressure", "bytes_per_point": 20, "ms": pw_ms,
+            "GBps": 20 * npt / (pw_ms / 1e3) / 1e9},
     }
 
 
@@ -469,6 +488,8 @@ def main():
     ap.add_argument("--batch-shots", type=int, default=32)
     ap.add_argument("--batch-nt", type=int, default=1000)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient leg")
+    ap.add_argument("--no-pointwise-ref", dest="pointwise_ref", action="store_false",
+                    help="skip the pointwise streaming reference (updatePressure on the same grid)")
     ap.add_argument("--no-plain", action="store_true", help="adjoint leg without the plain PyTorch timing")
     ap.add_argument("--adj-n", type=int, default=200)
     ap.add_argument("--adj-nt", type=int, default=1000)
@@ -522,7 +543,8 @@ def main():
         "batch": bat,
         "extra": {"fused_kernel_ms": round(gpu["kernel_ms"], 4),
                   "hpcscan_apparent_GBps": round(gpu["value"] * (pts_stencil + 4) * 4, 1),
-                  "device": gpu["device"]["name"], "arch": gpu["device"]["arch"]},
+                  "device": gpu["device"]["name"], "arch": gpu["device"]["arch"],
+                  "pointwise_ref": gpu["pointwise_ref"]},
     }
     print(json.dumps(out), flush=True)
 

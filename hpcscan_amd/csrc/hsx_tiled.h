@@ -321,17 +321,22 @@ __device__ __forceinline__ void tiled_body(const TiledArgs<TC> &a, int logical)
             if (hrow_ok[k]) *ring_h_at(slot, k) = hrow[k];
         if (hc_ok) *ring_c_at(slot) = hcol;
     };
-    auto load_streams = [&](int64_t z) {
+    // FP32 fused step of orders 6-10: prn and coef are loaded two planes ahead
+    // (registers to spare at 1024 threads; more bytes in flight per CU)
+    constexpr bool DEEP = MODE == MODE_AC2 && sizeof(TS) == 4 && sizeof(TC) == 4 && R >= 3 && R <= 5;
+    V nprn2 = vzero<TS, VW>(), ncoef2 = vzero<TS, VW>();   // plane z+2 (DEEP)
+    auto load_streams_to = [&](int64_t z, V &vp, V &vc) {
         if (MODE == MODE_AC2 && col_ok) {
             if (a.nt) {
-                nprn = vload_nt<TS, VW>(at(pb, z, col));
-                ncoef = vload_nt<TS, VW>(at(cb, z, col));
+                vp = vload_nt<TS, VW>(at(pb, z, col));
+                vc = vload_nt<TS, VW>(at(cb, z, col));
             } else {
-                nprn = vload<TS, VW>(at(pb, z, col));
-                ncoef = vload<TS, VW>(at(cb, z, col));
+                vp = vload<TS, VW>(at(pb, z, col));
+                vc = vload<TS, VW>(at(cb, z, col));
             }
         }
     };
+    auto load_streams = [&](int64_t z) { load_streams_to(z, nprn, ncoef); };
     if (RING) {
         // halos of planes z0 .. z0+R-1 into the ring, plane z0+R in registers
         for (int j = 0; j < R; j++) {
@@ -344,6 +349,9 @@ __device__ __forceinline__ void tiled_body(const TiledArgs<TC> &a, int logical)
         load_halo(z0);
     }
     load_streams(z0);
+    if constexpr (DEEP) {
+        if (z0 + 1 < z1) load_streams_to(z0 + 1, nprn2, ncoef2);
+    }
 
     // compute mask: which of the VW points of this lane are inner points
     bool inner[VW];
@@ -428,6 +436,10 @@ __device__ __forceinline__ void tiled_body(const TiledArgs<TC> &a, int logical)
             const V cur = q[cz];
             V prn_v = nprn;
             const V coef_v = ncoef;
+            if constexpr (DEEP) {
+                nprn = nprn2;
+                ncoef = ncoef2;
+            }
             if (fold & FOLD_PRN) zero_fold(prn_v, fl & 0xff, (fl >> 8) & 3, z);
 
             // stage plane z in LDS: centre rows from the queue, halo from the
@@ -462,7 +474,11 @@ __device__ __forceinline__ void tiled_body(const TiledArgs<TC> &a, int logical)
                 if (K::NEED_Z && col_ok) front = vload<TS, VW>(at(ub, z + 1 + R, col));
                 if (!K::NEED_Z && col_ok) q[(s + 1 + R) % P] = vload<TS, VW>(at(ub, z + 1, col));
                 load_halo(RING ? z + 1 + R : z + 1);
-                load_streams(z + 1);
+                if constexpr (DEEP) {
+                    if (z + 2 < z1) load_streams_to(z + 2, nprn2, ncoef2);
+                } else {
+                    load_streams(z + 1);
+                }
             }
 
             if (K::NEED_LDS) __syncthreads();
