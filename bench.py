@@ -160,8 +160,9 @@ def gpu_leg(args):
         raise RuntimeError(f"wavefield is not finite after the run (sum|p| = {s})")
 
     # pointwise reference on the same grid: updatePressure (prn = 2 prc - prn +
-    # coef * lapla, 4 reads + 1 write, no stencil, one-shot row launch) -- what
-    # plain streaming of this layout reaches on this device
+    # coef * lapla, 4 reads + 1 write, no stencil, one thread per 16 bytes in
+    # address order, nontemporal) -- what plain streaming of this layout
+    # reaches on this device
     pw_ms = None
     if args.pointwise_ref:
         lapla = capi.DeviceArray(nbytes)
@@ -196,7 +197,7 @@ def gpu_leg(args):
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "npt": npt, "device": info, "dt": float(dt),
         "pointwise_ref": None if pw_ms is None else {
-            "kernel": "k_update_pressure", "bytes_per_point": 20, "ms": pw_ms,
+            "kernel": "k_update_pressure16", "bytes_per_point": 20, "ms": pw_ms,
             "GBps": 20 * npt / (pw_ms / 1e3) / 1e9},
     }
 

@@ -310,6 +310,40 @@ __global__ void __launch_bounds__(256) k_update_pressure(TS *prn, const TS *prc,
     });
 }
 
+// The same update with 16-byte lanes (n1 a multiple of the vector width, 16-byte
+// aligned grids): one thread per aligned group of a row's [s1, e1] span, 256
+// threads per block over the (row, group) space in address order; points of an
+// edge group outside the box are written back with the prn value they were
+// read with. Every grid is read once and prn written once, so the accesses are
+// nontemporal. At 1000^3 FP32 this measured 3.64-3.70 ms against 3.95 ms for
+// the row kernel above (wave-per-row 16-byte lanes: 3.86-3.91 ms cached,
+// 4.16-4.20 ms nontemporal).
+template <typename TS, typename TC>
+__global__ void __launch_bounds__(256) k_update_pressure16(TS *prn, const TS *prc, const TS *coef, const TS *lapla,
+                                                           Box b, int64_t n1, int64_t n12, int64_t ng, int64_t total)
+{
+    constexpr int VW = 16 / sizeof(TS);
+    using NV = TS __attribute__((ext_vector_type(VW)));
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= total) return;
+    const int64_t row = t / ng, k = t - row * ng, N2 = b.n2();
+    const int64_t i2 = b.s2 + row % N2, i3 = b.s3 + row / N2;
+    const int64_t i1 = (b.s1 / VW + k) * VW, ii = i2 * n1 + i3 * n12 + i1;
+    const TC two = (TC)2.0;
+    const NV p = __builtin_nontemporal_load(reinterpret_cast<const NV *>(prn + ii));
+    const NV c = __builtin_nontemporal_load(reinterpret_cast<const NV *>(prc + ii));
+    const NV co = __builtin_nontemporal_load(reinterpret_cast<const NV *>(coef + ii));
+    const NV la = __builtin_nontemporal_load(reinterpret_cast<const NV *>(lapla + ii));
+    NV r;
+#pragma unroll
+    for (int v = 0; v < VW; v++) {
+        const TS cl = co[v] * la[v];
+        const TS nv = (TS)(two * (TC)c[v] - (TC)p[v] + (TC)cl);
+        r[v] = (i1 + v >= b.s1 && i1 + v <= b.e1) ? nv : p[v];
+    }
+    __builtin_nontemporal_store(r, reinterpret_cast<NV *>(prn + ii));
+}
+
 // ---- reductions (src/grid.cpp:2006-2370) ----------------------------------------
 enum RedOp { RED_SUM_ABS, RED_SUM_ABS_DIFF, RED_L1, RED_MAX_ERR, RED_MIN_MAX };
 
@@ -574,6 +608,17 @@ int hsx_update_pressure(const hsx_grid *g, void *prn, const void *prc, const voi
     return dispatch_types(g, [&](auto ts, auto tc) -> int {
         using TS = decltype(ts);
         using TC = decltype(tc);
+        constexpr int VW = 16 / sizeof(TS);
+        const bool wide = g->n1 % VW == 0 &&
+                          ((uintptr_t)prn | (uintptr_t)prc | (uintptr_t)coef | (uintptr_t)lapla) % 16 == 0;
+        const int64_t ng = b.e1 / VW - b.s1 / VW + 1, total = ng * b.n2() * b.n3();
+        if (wide && total > 0 && (total + 255) / 256 <= INT32_MAX) {
+            hipLaunchKernelGGL((k_update_pressure16<TS, TC>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                               (TS *)prn, (const TS *)prc, (const TS *)coef, (const TS *)lapla, b, g->n1,
+                               g->n1 * g->n2, ng, total);
+            HSX_LAUNCHED("k_update_pressure16");
+            return HSX_OK;
+        }
         hipLaunchKernelGGL((k_update_pressure<TS, TC>), row_grid(b), dim3(ROW_THREADS, ROWS_PER_BLOCK), 0, s,
                            (TS *)prn, (const TS *)prc, (const TS *)coef, (const TS *)lapla, b, g->n1,
                            g->n1 * g->n2);

@@ -106,6 +106,29 @@ def test_stencils_bit_exact(oracle, capi, family, dim, n, order, prec):
     _same(dp.download(dtype), ref)
 
 
+@pytest.mark.parametrize("prec", list(PRECS))
+@pytest.mark.parametrize("dim,n,order,mul", [
+    (3, (21, 18, 17), 2, 4), (3, (21, 18, 17), 6, 8), (3, (9, 10, 11), 8, 4), (3, (40, 6, 5), 16, 4),
+    (2, (23, 19), 4, 4), (1, (37,), 10, 8), (3, (21, 18, 17), 8, 0)])
+def test_update_pressure_wide_bit_exact(oracle, capi, dim, n, order, mul, prec):
+    """updatePressure on grids whose n1 is a multiple of the 16-byte vector
+    (n1MulPad), where the 16-byte-lane kernel runs and the edge groups of each
+    row hold points outside the box; mul 0: the reference's own n1 padding"""
+    dtype, c64 = PRECS[prec]
+    mp = (mul, -1, -1) if mul else (-1, -1, -1)
+    p, o = _layouts(oracle, dim, n, order, dtype, mul_pad=mp)
+    print(f"case dim={dim} n={n} order={order} prec={prec} n1={p.n1} inner={p.inner()}", flush=True)
+    rng = np.random.default_rng(7 * order + dim)
+    g = capi.hsx_grid(p, compute_fp64=c64)
+    u, coef, prn0, lap = (_rand(rng, p.npoint, dtype) for _ in range(4))
+    ref = prn0.copy()
+    oracle.update_pressure(o, ref, u, coef, lap, compute64=c64)
+    dp = capi.DeviceArray.from_numpy(prn0)
+    du, dc, dl = (capi.DeviceArray.from_numpy(x) for x in (u, coef, lap))
+    capi.update_pressure(g, dp, du, dc, dl)
+    _same(dp.download(dtype), ref)
+
+
 @pytest.mark.parametrize("dim,n", [(1, (30,)), (2, (17, 20)), (3, (12, 11, 13))])
 @pytest.mark.parametrize("order", [2, 8, 16])
 @pytest.mark.parametrize("prec", ["f32", "f64"])
