@@ -821,13 +821,17 @@ inline int launch_tiled_by(const hsx_grid *g, void *out, const void *prn_in, con
     if constexpr (TallFits<TS, TC, R, VW, LX, MODE>::tall) {
         if (tall == 2) return launch_tiled<TS, TC, R, VW, LX, 16, MODE, false, FOLD>(g, out, prn_in, u, coef, s);
     }
-    // 8 rows: the ring only when 8 rows are asked for (at 1000^3 FP32 order 8
-    // it measured 3.83 ms against 3.66 ms without it: its LDS halves the
-    // resident workgroups while the 8-row halo stays large)
+    // 8 rows: the ring by default for FP64 storage only (1000^3 fused step,
+    // 64 x 8: FP64 6.88 / 7.26 / 7.48 ms at orders 4 / 8 / 12 with the ring and
+    // nontemporal streams, 6.99 / 7.36 / 7.78 without; FP32 order 8 3.83 ms
+    // with against 3.66 ms without, FP32 storage with FP64 compute 3.89 against
+    // 3.70: there its LDS halves the resident workgroups while the 8-row halo
+    // stays large); otherwise only when 8 rows are asked for
     constexpr bool ring8_ok = Tiled<TS, TC, R, VW, LX, 8, MODE, true>::RING &&
                               Tiled<TS, TC, R, VW, LX, 8, MODE, true>::LDS <= TILE_LDS_MAX;
     if constexpr (ring8_ok && !FOLD) {
-        if (g_tile.by == 8 && !(g_tile.flags & HSX_TILE_NO_RING))
+        const bool ring8 = g_tile.by == 8 || (g_tile.by == 0 && sizeof(TS) == 8);
+        if (ring8 && !(g_tile.flags & HSX_TILE_NO_RING))
             return launch_tiled<TS, TC, R, VW, LX, 8, MODE, true>(g, out, prn_in, u, coef, s);
     }
     return launch_tiled<TS, TC, R, VW, LX, 8, MODE, false, FOLD>(g, out, prn_in, u, coef, s);
@@ -862,8 +866,15 @@ inline int run_tiled_shape(const hsx_grid *g, void *out, const void *prn_in, con
         return dispatch_R(R, [&](auto rc_) -> int {
             constexpr int RR = decltype(rc_)::value;
             if (lx == 16) return launch_tiled<TS, TC, RR, VW, 16, 8, MODE, false, FOLD>(g, out, prn_in, u, coef, s);
-            // 64 lanes whose 16-row instances all spill (order 16 FP32): 32 lanes x 16 rows
-            const bool narrow = lx == 64 && g_tile.lx == 0 && tall_variant<TS, TC, RR, VW, 64, MODE, FOLD>() == 0 &&
+            // 64 lanes without a 16-row instance that keeps its values in
+            // registers: FP64 values up to order 12 take 64 x 8 tiles, the rest
+            // (order 16 FP32, orders 14-16 in FP64) 32 lanes x 16 rows.
+            // 1000^3 FP64 order 8: 64 x 8 7.26 ms, 32 x 16 7.96 ms; order 16:
+            // 7.73 against 7.47 ms. FP32 storage with FP64 compute, order 8:
+            // 3.70 against 4.27 ms; FP32 order 16: 4.09 against 3.90 ms
+            constexpr bool fp64_vals = !(sizeof(TS) == 4 && sizeof(TC) == 4);
+            const bool narrow = lx == 64 && g_tile.lx == 0 && !(fp64_vals && RR <= 6) &&
+                                tall_variant<TS, TC, RR, VW, 64, MODE, FOLD>() == 0 &&
                                 tall_variant<TS, TC, RR, VW, 32, MODE, FOLD>() != 0;
             if (lx == 32 || narrow) return launch_tiled_by<TS, TC, RR, VW, 32, MODE, FOLD>(g, out, prn_in, u, coef, s);
             return launch_tiled_by<TS, TC, RR, VW, 64, MODE, FOLD>(g, out, prn_in, u, coef, s);
