@@ -723,9 +723,10 @@ inline bool tiled_spills(KERN kern)
 }
 
 // kernel arguments of one grid; slots = resident workgroups available to it
+// (chunk > 0: n3 planes per workgroup, at least 4R, unless hsx_set_tile forces one)
 template <typename TS, typename TC, int R, int VW, int LX, int BY, int MODE, bool RING = false>
 inline int fill_tiled_args(const hsx_grid *g, void *out, const void *prn_in, const void *u, const void *coef,
-                           int64_t slots, TiledArgs<TC> &a)
+                           int64_t slots, TiledArgs<TC> &a, int64_t chunk = 0)
 {
     a.out = out;
     a.prn_in = prn_in ? prn_in : out;
@@ -749,6 +750,7 @@ inline int fill_tiled_args(const hsx_grid *g, void *out, const void *prn_in, con
     const int64_t N3 = a.e3 - a.s3 + 1;
     const int64_t tiles = (int64_t)a.ntx * a.nty;
     const int64_t len = g_tile.chunk > 0 ? std::min<int64_t>(g_tile.chunk, N3)
+                        : chunk > 0           ? std::min<int64_t>(std::max<int64_t>(chunk, 4 * R), N3)
                         : (BY == 16 || RING) ? chunk_one_round(N3, R, tiles, slots)
                                           : chunk_short(N3, R, tiles);
     a.chunk = (int)len;

@@ -199,3 +199,25 @@ def test_batched_shots_equal_single_shots(oracle, capi, fp64, use_graph):
             ref = oracle.forward(sh["o"], sh["coef"], sh["iv"], sh["nt"], sh["src"], sh["wavelet"], sh["rec"])
             _same(tr.download(sh["dtype"]).reshape(ref.shape), ref)
     b.close()
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_batched_shots_several_rounds(oracle, capi, fp64):
+    """a batch whose tiles outnumber the resident workgroups (16 shots of 800
+    rows: 50 tile rows each), so the launch streams short n3 chunks in several
+    rounds, with uneven last chunks (N3 = 36..66): traces equal the oracle's"""
+    sizes = [(16 + 4 * (k % 3), 800, 36 + 2 * k) for k in range(16)]
+    shots = []
+    for k, n in enumerate(sizes):
+        sh = make_shot(oracle, n, 8, nt=10, fp64=fp64, seed=90 + k, with_obs=False)
+        d, s = _shot_dev(capi, sh)
+        tr = capi.DeviceArray(sh["nt"] * len(sh["rec"]) * np.dtype(sh["dtype"]).itemsize)
+        shots.append((sh, d, s, tr))
+    print(f"batch of {len(sizes)} shots, fp64={fp64}", flush=True)
+    b = capi.Batch([capi.hsx_grid(x[0]["p"]) for x in shots], [x[1]["coef"] for x in shots],
+                   [x[2] for x in shots], [x[3] for x in shots])
+    b.run(use_graph=False)
+    for sh, d, s, tr in shots:
+        ref = oracle.forward(sh["o"], sh["coef"], sh["iv"], sh["nt"], sh["src"], sh["wavelet"], sh["rec"])
+        _same(tr.download(sh["dtype"]).reshape(ref.shape), ref)
+    b.close()
