@@ -663,11 +663,20 @@ struct FoldReq {
 extern thread_local FoldReq t_fold;   // defined in hsx_tiled_ac2.hip
 constexpr int TILE_LDS_MAX = 163840;  // gfx950: 160 KiB of LDS per workgroup
 
+// 64 lanes unless the tiles of 64 lanes span more than 5/4 of the n1 width
+// that tiles of 32 lanes span (their last tile mostly idle): 328^3 FP32 order
+// 8 ran 0.125 ms with 32 x 16 tiles against 0.150 ms with 64 x 16 (512 lanes'
+// width for 320 points); 608^3 and 808^3 stay faster with 64 lanes (0.75 /
+// 1.81 ms against 0.93 / 2.22 ms) although a fifth of their lanes idle
 inline int pick_lx(const hsx_grid *g, int vw)
 {
     if (g_tile.lx) return g_tile.lx;
     const int64_t N1 = g->inner[1] - g->inner[0] + 1;
-    return N1 <= 16 * vw ? 16 : N1 <= 32 * vw ? 32 : 64;
+    if (N1 <= 16 * vw) return 16;
+    if (N1 <= 32 * vw) return 32;
+    const int64_t w64 = (N1 + 64 * vw - 1) / (64 * vw) * (64 * vw);
+    const int64_t w32 = (N1 + 32 * vw - 1) / (32 * vw) * (32 * vw);
+    return 4 * w64 > 5 * w32 ? 32 : 64;
 }
 
 // 8-row tiles: chunks of ~48 planes (at least 4R), at least 2048 workgroups
