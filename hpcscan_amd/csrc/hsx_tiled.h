@@ -898,7 +898,9 @@ inline int run_tiled_shape(const hsx_grid *g, void *out, const void *prn_in, con
             // (order 16 FP32, orders 14-16 in FP64) 32 lanes x 16 rows.
             // 1000^3 FP64 order 8: 64 x 8 7.26 ms, 32 x 16 7.96 ms; order 16:
             // 7.73 against 7.47 ms. FP32 storage with FP64 compute, order 8:
-            // 3.70 against 4.27 ms; FP32 order 16: 4.09 against 3.90 ms
+            // 3.70 against 4.27 ms; FP32 order 16: 4.09 against 3.90 ms (and
+            // 4.50 ms for 64 x 16 with 8-byte lanes, which fits in registers
+            // but issues twice the loads and LDS reads per point)
             constexpr bool fp64_vals = !(sizeof(TS) == 4 && sizeof(TC) == 4);
             const bool narrow = lx == 64 && g_tile.lx == 0 && !(fp64_vals && RR <= 6) &&
                                 tall_variant<TS, TC, RR, VW, 64, MODE, FOLD>() == 0 &&

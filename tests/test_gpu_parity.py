@@ -355,6 +355,9 @@ TILED_CASES = [
     (16, "f32", 140, 9, 20, 64, 0, 8, 0),
     (16, "f32", 140, 41, 20, 32, 0, 16, 2),   # R = 8 at 16 rows with the ring
     (16, "f32", 270, 41, 20, 64, 0, 16, 0),   # R = 8, 64 lanes: ring too large, 16 rows without it
+    (16, "f32", 200, 41, 20, 0, 0, 0, 0),     # default shape at R = 8 FP32, 64-lane width: 32 x 16 ring
+    (14, "f32", 196, 37, 23, 0, 0, 0, 0),     # the same at R = 7 (odd: n1 offset)
+    (16, "f32", 500, 19, 37, 0, 0, 0, 0),     # four 128-point tiles per row, last one partial
     (4, "f64", 70, 21, 27, 16, 5, 0, 0),      # VW = 2
     (4, "f64", 150, 37, 27, 64, 9, 16, 2),
     (4, "f64", 150, 37, 27, 64, 9, 16, 4),
