@@ -1,6 +1,10 @@
 // Data acquisition on the MI355X backend: the source term and the receivers
 // work on the device grid (hsx_inject / hsx_gather); traces stay in HBM,
-// time-major, until writeTrace copies them back once.
+// time-major, until writeTrace copies them back once. Both are queued on the
+// library stream without waiting: the next Grid call on that stream (which
+// completes before it returns) or writeTrace's copy orders them, so the time
+// loop pays no host round trip for them (400^3 order 8 modeling: the time loop
+// ran at 163.9 GPoint/s when each of them waited for the stream).
 #include <vector>
 
 #include "data_Acquisition.h"
@@ -36,7 +40,7 @@ public:
                 "hsx_inject"))
             return RTN_CODE_KO;
         g->touched();
-        return ok(hsx_stream_sync(nullptr), "hsx_stream_sync") ? RTN_CODE_OK : RTN_CODE_KO;
+        return RTN_CODE_OK;
     }
 
     Rtn_code recordTrace(Grid &grid, Modeling_type modType, Myint it, Myfloat dt) override
@@ -51,8 +55,7 @@ public:
         }
         hsx_grid d = g->desc();
         char *out = static_cast<char *>(d_traces) + (size_t)it * nRecLoc * sizeof(Myfloat);
-        if (!ok(hsx_gather(&d, g->d_grid, nRecLoc, static_cast<const int64_t *>(d_rec), out, nullptr), "hsx_gather") ||
-            !ok(hsx_stream_sync(nullptr), "hsx_stream_sync"))
+        if (!ok(hsx_gather(&d, g->d_grid, nRecLoc, static_cast<const int64_t *>(d_rec), out, nullptr), "hsx_gather"))
             return RTN_CODE_KO;
         record_trace_time = hpcWtime() - t0;
         return RTN_CODE_OK;
