@@ -149,31 +149,31 @@ __global__ void __launch_bounds__(256) k_free_surface(TS *u, Face f)
 // A halo point is written once and mirrors read only inner points that the
 // first launch does not write, so the first launch has no ordering inside it,
 // and the second launch (the planes) only writes inner points.
-template <typename TS>
-__device__ __forceinline__ void bc_mirror_face(TS *u, const FaceSet &fs, int face)
+// The face loops index with IT = uint32_t when the face fits (always, below
+// 65536^2 points per face): the 64-bit divisions otherwise dominate these
+// kernels, which move only a few bytes per index.
+template <typename TS, typename IT>
+__device__ __forceinline__ void bc_mirror_loop(TS *u, const FaceSet &fs, const Face &f, IT H, IT NB, IT total)
 {
-    const Face &f = fs.f[face];
     const int a = f.axis;
     const int oa = a == 2 ? 1 : 2, ob = a == 0 ? 1 : 0;
-    const int64_t H = f.he - f.hs + 1;
-    const int64_t NB = f.be - f.bs + 1, NO = f.oe - f.os + 1;
-    const int64_t total = H * NB * NO;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        int64_t k, ib, io;
+    for (IT t = (IT)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (IT)gridDim.x * blockDim.x) {
+        IT k, ib, io;
         if (a == 0) {
-            k = t % H;
-            ib = (t / H) % NB;
-            io = t / (H * NB);
+            const IT r = t / H;
+            k = t - r * H;
+            io = r / NB;
+            ib = r - io * NB;
         } else {
-            ib = t % NB;
-            k = (t / NB) % H;
-            io = t / (NB * H);
+            const IT r = t / NB;
+            ib = t - r * NB;
+            io = r / H;
+            k = r - io * H;
         }
         int64_t c[3];
-        c[a] = f.hs + k;
-        c[oa] = f.os + io;
-        c[ob] = f.bs + ib;
+        c[a] = f.hs + (int64_t)k;
+        c[oa] = f.os + (int64_t)io;
+        c[ob] = f.bs + (int64_t)ib;
         bool zero = false;
         for (int ap = 0; ap < a; ap++)
             for (int sp = 0; sp < 2; sp++)
@@ -185,16 +185,37 @@ __device__ __forceinline__ void bc_mirror_face(TS *u, const FaceSet &fs, int fac
 }
 
 template <typename TS>
+__device__ __forceinline__ void bc_mirror_face(TS *u, const FaceSet &fs, int face)
+{
+    const Face &f = fs.f[face];
+    const int64_t H = f.he - f.hs + 1;
+    const int64_t NB = f.be - f.bs + 1, NO = f.oe - f.os + 1;
+    const int64_t total = H * NB * NO;
+    if (total <= (int64_t)UINT32_MAX - (int64_t)gridDim.x * blockDim.x)
+        bc_mirror_loop<TS, uint32_t>(u, fs, f, (uint32_t)H, (uint32_t)NB, (uint32_t)total);
+    else
+        bc_mirror_loop<TS, uint64_t>(u, fs, f, (uint64_t)H, (uint64_t)NB, (uint64_t)total);
+}
+
+template <typename TS, typename IT>
+__device__ __forceinline__ void bc_zero_loop(TS *u, const Face &f, IT NB, IT total)
+{
+    for (IT t = (IT)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (IT)gridDim.x * blockDim.x) {
+        const IT io = t / NB, ib = t - io * NB;
+        u[(f.os + (int64_t)io) * f.so + (f.bs + (int64_t)ib) * f.sb + f.inner * f.sa] = (TS)0.0;
+    }
+}
+
+template <typename TS>
 __device__ __forceinline__ void bc_zero_face(TS *u, const FaceSet &fs, int face)
 {
     const Face &f = fs.f[face];
     const int64_t NB = f.be - f.bs + 1, NO = f.oe - f.os + 1;
     const int64_t total = NB * NO;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t ib = t % NB, io = t / NB;
-        u[(f.os + io) * f.so + (f.bs + ib) * f.sb + f.inner * f.sa] = (TS)0.0;
-    }
+    if (total <= (int64_t)UINT32_MAX - (int64_t)gridDim.x * blockDim.x)
+        bc_zero_loop<TS, uint32_t>(u, f, (uint32_t)NB, (uint32_t)total);
+    else
+        bc_zero_loop<TS, uint64_t>(u, f, (uint64_t)NB, (uint64_t)total);
 }
 
 // one grid: blockIdx.y = face

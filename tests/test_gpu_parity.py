@@ -358,6 +358,9 @@ TILED_CASES = [
     (16, "f32", 200, 41, 20, 0, 0, 0, 0),     # default shape at R = 8 FP32, 64-lane width: 32 x 16 ring
     (14, "f32", 196, 37, 23, 0, 0, 0, 0),     # the same at R = 7 (odd: n1 offset)
     (16, "f32", 500, 19, 37, 0, 0, 0, 0),     # four 128-point tiles per row, last one partial
+    (8, "f32", 300, 20, 20, 0, 0, 0, 0),      # default at 300 points per row: 32-lane tiles (three of 128)
+    (8, "f64", 250, 21, 19, 0, 0, 0, 0),      # default FP64 at order 8: 64 x 8 with the ring, nontemporal
+    (8, "f32c64", 250, 21, 19, 0, 0, 0, 0),   # FP32 storage, FP64 compute: 64 x 8 without the ring
     (4, "f64", 70, 21, 27, 16, 5, 0, 0),      # VW = 2
     (4, "f64", 150, 37, 27, 64, 9, 16, 2),
     (4, "f64", 150, 37, 27, 64, 9, 16, 4),
