@@ -156,12 +156,14 @@ def test_free_surface_small_grids(oracle, capi, n, order):
     _same(d.download(np.float32), ref)
 
 
-@pytest.mark.parametrize("mask", [0x01, 0x02, 0x0C, 0x30, 0x15, 0x2A, 0x3E, 0x1F, 0x00])
+@pytest.mark.parametrize("mask", [0x01, 0x02, 0x0C, 0x30, 0x15, 0x2A, 0x3E, 0x1F, 0x03, 0x00])
 @pytest.mark.parametrize("prec", ["f32", "f64"])
-def test_free_surface_face_masks(oracle, capi, mask, prec):
-    """faces with a neighbour left out (multi-rank decomposition)"""
+@pytest.mark.parametrize("n", [(14, 12, 13), (16, 12, 13)])
+def test_free_surface_face_masks(oracle, capi, mask, prec, n):
+    """faces with a neighbour left out (multi-rank decomposition); n1 = 24 in
+    FP32 takes the 16-byte n1-face kernel, n1 = 22 the per-point one"""
     dtype, _ = PRECS[prec]
-    p, o = _layouts(oracle, 3, (14, 12, 13), 8, dtype)
+    p, o = _layouts(oracle, 3, n, 8, dtype)
     u = _rand(np.random.default_rng(mask), p.npoint, dtype)
     ref = u.copy()
     oracle.free_surface(o, ref, face_mask=mask)
