@@ -61,8 +61,11 @@ struct hsx_batch {
     void *npoint = nullptr;                    // [2*nshots]
     int64_t most_npoint = 0;
     void *bc_fields[2] = {nullptr, nullptr};   // [nshots] the wavefield B applies to, per parity
-    void *sets = nullptr;                      // FaceSet per shot
+    void *sets = nullptr;                      // FaceSet per shot (without the n1 faces k_bc_n1 takes)
     int64_t most = 0, most_plane = 0;
+    void *n1 = nullptr;                        // N1Faces per shot (k_bc_n1), when n1_nh > 0
+    int n1_nh = 0;                             // R / vector width
+    int64_t n1_rows = 0;                       // largest number of rows of an n1 face
     void *sr[2] = {nullptr, nullptr};          // SrcRec per shot, per parity
     std::function<int(int, hipStream_t)> step; // batched fused step, per parity
     hipGraph_t graph = nullptr;
@@ -183,7 +186,12 @@ static int batch_record(hsx_batch *b, hipStream_t s)
     HSX_LAUNCHED("k_zero_batch");
     for (int64_t n = 0; n < b->nt; n++) {
         const int p = (int)(n & 1);
-        int rc = bc_batch_launch(b->storage, ns, (void *const *)b->bc_fields[p], (const FaceSet *)b->sets, b->most,
+        int rc = HSX_OK;
+        if (b->n1_nh)
+            rc = bc_batch_n1_launch(b->storage, ns, (void *const *)b->bc_fields[p], (const N1Faces *)b->n1, b->n1_nh,
+                                    b->n1_rows, s);
+        if (!rc)
+            rc = bc_batch_launch(b->storage, ns, (void *const *)b->bc_fields[p], (const FaceSet *)b->sets, b->most,
                                  b->most_plane, s);
         if (!rc) rc = b->step(p, s);
         if (rc) return rc;
@@ -231,6 +239,7 @@ int hsx_batch_create(int nshots, const hsx_grid *grids, const void *const *coef,
     std::vector<void *> F(2 * nshots);
     std::vector<int64_t> np(2 * nshots);
     std::vector<FaceSet> sets(nshots);
+    std::vector<N1Faces> n1(nshots);
     std::vector<SrcRec> sr[2];
     sr[0].resize(nshots);
     sr[1].resize(nshots);
@@ -256,6 +265,19 @@ int hsx_batch_create(int nshots, const hsx_grid *grids, const void *const *coef,
         if (!closed) return bail(fail(HSX_ERR_UNSUPPORTED, "batch: shot %d too small for the two-launch boundary", s));
         b->most = std::max(b->most, most);
         b->most_plane = std::max(b->most_plane, most_plane);
+        // n1 faces in one 16-byte pass where they qualify; the rest stay with
+        // the generic face kernels (the boundary-plane set zmask / zidx is kept)
+        N1Faces &x = n1[s];
+        x.sides = n1_vector_sides(&g, A, sets[s], &x.lo, &x.hi);
+        if (x.sides) {
+            const Face &f = (x.sides & 1) ? x.lo : x.hi;
+            b->n1_rows = std::max(b->n1_rows, (f.be - f.bs + 1) * (f.oe - f.os + 1));
+            b->n1_nh = (int)((f.he - f.hs + 1) / (g.storage == HSX_FP64 ? 2 : 4));
+            int k = 0;
+            for (int j = 0; j < sets[s].n; j++)
+                if (sets[s].f[j].axis != 0) sets[s].f[k++] = sets[s].f[j];
+            sets[s].n = k;
+        }
         // parity 0: free surface on B, step into A; parity 1: the other way round
         bcf[0][s] = B;
         bcf[1][s] = A;
@@ -274,6 +296,7 @@ int hsx_batch_create(int nshots, const hsx_grid *grids, const void *const *coef,
         (rc = upload(b, np.data(), np.size() * sizeof(int64_t), &b->npoint)) ||
         (rc = upload(b, sets.data(), sets.size() * sizeof(FaceSet), &b->sets)))
         return bail(rc);
+    if (b->n1_nh && (rc = upload(b, n1.data(), n1.size() * sizeof(N1Faces), &b->n1))) return bail(rc);
     for (int p = 0; p < 2; p++)
         if ((rc = upload(b, bcf[p].data(), nshots * sizeof(void *), &b->bc_fields[p])) ||
             (rc = upload(b, sr[p].data(), nshots * sizeof(SrcRec), &b->sr[p])))

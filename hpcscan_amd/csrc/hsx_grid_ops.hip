@@ -229,14 +229,13 @@ __device__ __forceinline__ void bc_zero_face(TS *u, const FaceSet &fs, int face)
 // is stored back as read. 1000^3 FP32 order 8: 0.076 ms for both n1 faces,
 // against 0.097 ms for per-point mirrors plus their zero launch.
 template <typename TS, int NH>
-__global__ void __launch_bounds__(256) k_bc_n1(TS *u, Face lo, Face hi, int sides)
+__device__ __forceinline__ void bc_n1_rows(TS *u, const Face &lo, const Face &hi, int sides, uint32_t t)
 {
     constexpr int VW = 16 / sizeof(TS), R = NH * VW;
     using NV = TS __attribute__((ext_vector_type(VW)));
     const Face &f = (sides & 1) ? lo : hi;           // both faces span the same rows
     const uint32_t NB = (uint32_t)(f.be - f.bs + 1), rows = NB * (uint32_t)(f.oe - f.os + 1);
     const uint32_t ns = (uint32_t)(sides & 1) + (uint32_t)(sides >> 1);
-    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     if (t >= rows * ns) return;
     const uint32_t row = ns == 2 ? t >> 1 : t;
     const int side = ns == 2 ? (int)(t & 1) : ((sides & 1) ? 0 : 1);
@@ -281,11 +280,23 @@ __global__ void __launch_bounds__(256) k_bc_n1(TS *u, Face lo, Face hi, int side
     }
 }
 
-// whether the n1 faces of fs take k_bc_n1 (sides bit 0 / 1: low / high face)
-template <typename TS>
-static int n1_vector_sides(const hsx_grid *g, const void *u, const FaceSet &fs, Face *lo, Face *hi)
+template <typename TS, int NH>
+__global__ void __launch_bounds__(256) k_bc_n1(TS *u, Face lo, Face hi, int sides)
 {
-    constexpr int VW = 16 / sizeof(TS);
+    bc_n1_rows<TS, NH>(u, lo, hi, sides, blockIdx.x * 256u + threadIdx.x);
+}
+
+// many grids: blockIdx.y = grid
+template <typename TS, int NH>
+__global__ void __launch_bounds__(256) k_bc_n1_batch(TS *const *u, const N1Faces *nf)
+{
+    const N1Faces &x = nf[blockIdx.y];
+    if (x.sides) bc_n1_rows<TS, NH>(u[blockIdx.y], x.lo, x.hi, x.sides, blockIdx.x * 256u + threadIdx.x);
+}
+
+int n1_vector_sides(const hsx_grid *g, const void *u, const FaceSet &fs, Face *lo, Face *hi)
+{
+    const int VW = g->storage == HSX_FP64 ? 2 : 4;
     if (g->n1 % VW || (uintptr_t)u % 16) return 0;
     int sides = 0;
     for (int k = 0; k < fs.n; k++) {
@@ -404,6 +415,29 @@ int bc_batch_launch(int storage, int nshots, void *const *d_fields, const FaceSe
                            (float *const *)d_fields, d_sets);
     }
     HSX_LAUNCHED("k_bc_zero_batch");
+    return HSX_OK;
+}
+
+int bc_batch_n1_launch(int storage, int nshots, void *const *d_fields, const N1Faces *d_n1, int nh, int64_t most_rows,
+                       hipStream_t s)
+{
+    const dim3 grid((unsigned)std::max<int64_t>(1, (2 * most_rows + 255) / 256), (unsigned)nshots);
+    if (storage == HSX_FP64) {
+        double *const *u = (double *const *)d_fields;
+        switch (nh) {
+        case 1: hipLaunchKernelGGL((k_bc_n1_batch<double, 1>), grid, dim3(256), 0, s, u, d_n1); break;
+        case 2: hipLaunchKernelGGL((k_bc_n1_batch<double, 2>), grid, dim3(256), 0, s, u, d_n1); break;
+        case 3: hipLaunchKernelGGL((k_bc_n1_batch<double, 3>), grid, dim3(256), 0, s, u, d_n1); break;
+        default: hipLaunchKernelGGL((k_bc_n1_batch<double, 4>), grid, dim3(256), 0, s, u, d_n1); break;
+        }
+    } else {
+        float *const *u = (float *const *)d_fields;
+        switch (nh) {
+        case 1: hipLaunchKernelGGL((k_bc_n1_batch<float, 1>), grid, dim3(256), 0, s, u, d_n1); break;
+        default: hipLaunchKernelGGL((k_bc_n1_batch<float, 2>), grid, dim3(256), 0, s, u, d_n1); break;
+        }
+    }
+    HSX_LAUNCHED("k_bc_n1_batch");
     return HSX_OK;
 }
 
@@ -692,7 +726,7 @@ int hsx_free_surface(const hsx_grid *g, void *u, int face_mask, void *stream)
             const unsigned nb2 = (unsigned)std::min<int64_t>((most_plane + 255) / 256, 4096);
             FaceSet rest = fs;                   // the faces left to the two generic launches
             Face lo{}, hi{};
-            const int sides = n1_vector_sides<TS>(g, u, fs, &lo, &hi);
+            const int sides = n1_vector_sides(g, u, fs, &lo, &hi);
             if (sides) {
                 const Face &f = (sides & 1) ? lo : hi;
                 const int64_t threads = (f.be - f.bs + 1) * (f.oe - f.os + 1) * (sides == 3 ? 2 : 1);

@@ -71,6 +71,18 @@ int make_faceset(const hsx_grid *g, int face_mask, FaceSet *fs, bool *closed_for
 // free surface of many grids (closed form only): d_fields / d_sets device arrays
 int bc_batch_launch(int storage, int nshots, void *const *d_fields, const FaceSet *d_sets, int64_t most,
                     int64_t most_plane, hipStream_t s);
+// the n1 faces in one 16-byte pass per row end (k_bc_n1): sides (bit 0 / 1 = low /
+// high face, 0 = not applicable) when R is a multiple of the vector width, n1
+// a multiple of it, u 16-byte aligned and the halo groups aligned
+int n1_vector_sides(const hsx_grid *g, const void *u, const FaceSet &fs, Face *lo, Face *hi);
+struct N1Faces {
+    Face lo, hi;
+    int sides;                   // 0: this grid's n1 faces stay with the generic kernels
+};
+// many grids: d_n1 per grid, nh = R / vector width (the same for all), most_rows
+// = the largest number of rows of an n1 face
+int bc_batch_n1_launch(int storage, int nshots, void *const *d_fields, const N1Faces *d_n1, int nh, int64_t most_rows,
+                       hipStream_t s);
 
 // ---- coefficients in the compute type (src/fdm.h:276-507, src/grid.cpp:3251-3257) --
 template <typename TC>

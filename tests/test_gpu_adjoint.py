@@ -181,13 +181,16 @@ def masks_inner(p):
 
 @pytest.mark.parametrize("fp64", [False, True])
 @pytest.mark.parametrize("use_graph", [False, True])
-def test_batched_shots_equal_single_shots(oracle, capi, fp64, use_graph):
+@pytest.mark.parametrize("order", [8, 4])
+def test_batched_shots_equal_single_shots(oracle, capi, fp64, use_graph, order):
     """uneven grids in one batch (one fused-step launch per time step, optionally
-    a captured graph replayed twice): every shot's traces equal the oracle's"""
+    a captured graph replayed twice): every shot's traces equal the oracle's.
+    Order 8 (and FP64 order 4) takes the 16-byte n1-face kernel, FP32 order 4
+    (R = 2, not a multiple of the vector) the per-point face kernels"""
     sizes = [(36, 20, 18), (40, 24, 22), (28, 30, 26), (68, 36, 20)]
     shots, keep = [], []
     for k, n in enumerate(sizes):
-        sh = make_shot(oracle, n, 8, nt=30, fp64=fp64, seed=50 + k, with_obs=False)
+        sh = make_shot(oracle, n, order, nt=30, fp64=fp64, seed=50 + k, with_obs=False)
         d, s = _shot_dev(capi, sh)
         tr = capi.DeviceArray(sh["nt"] * len(sh["rec"]) * np.dtype(sh["dtype"]).itemsize)
         shots.append((sh, d, s, tr))
