@@ -226,8 +226,9 @@ __device__ __forceinline__ void bc_zero_face(TS *u, const FaceSet &fs, int face)
 // (those halo points are -0 by the same rule), so the zero may be written
 // here; the mirrors read old values of the other faces' boundary planes,
 // which k_bc_zero clears after this launch. The rest of each written group
-// is stored back as read. 1000^3 FP32 order 8: 0.076 ms for both n1 faces,
-// against 0.097 ms for per-point mirrors plus their zero launch.
+// is stored back as read. 1000^3 FP32 order 8: 0.075 ms for both n1 faces,
+// against 0.111 ms through k_bc_mirror + k_bc_zero (a second pass over the
+// same 64-byte sectors of 2 x 10^6 rows).
 template <typename TS, int NH>
 __device__ __forceinline__ void bc_n1_rows(TS *u, const Face &lo, const Face &hi, int sides, uint32_t t)
 {
