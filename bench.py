@@ -4,7 +4,7 @@
 One step is one time step of hpcscan's propagator (src/propagator_Ac2.cpp:366-393):
 free-surface boundary on prc, then the fused Laplacian + leapfrog on prn, over
 the 1000^3 inner grid (eigen mode 2,2,2 initial state, dt = CFL limit); the
-free surface runs as two launches of its own before each step (--fold: folded
+free surface runs as launches of its own before each step (--fold: folded
 into the fused kernel instead, slower on this device).  Grids are resident in
 HBM before timing.  Prints one JSON line.
 
@@ -480,7 +480,7 @@ def main():
     ap.add_argument("--streams", default="default", choices=["default", "cached", "nt"],
                     help="cached or nontemporal prn/coef/output accesses of the fused kernel")
     ap.add_argument("--fold", action="store_true",
-                    help="free surface folded into the fused step (hsx_ac2_step_fs) instead of its own two launches")
+                    help="free surface folded into the fused step (hsx_ac2_step_fs) instead of its own launches")
     ap.add_argument("--no-ring", action="store_true",
                     help="load each plane's halo one plane ahead instead of with the queue front")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -537,7 +537,7 @@ def main():
                    "inner_points": npt, "kernel": args.kernel,
                    "tile_lx": args.tile_lx, "tile_rows": args.tile_rows, "tile_chunk": args.tile_chunk,
                    "streams": args.streams, "halo_ring": not args.no_ring,
-                   "free_surface": "folded into the step" if args.fold else "two launches before the step"},
+                   "free_surface": "folded into the step" if args.fold else "own launches before the step"},
         "roofline": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in gpu["roofline"].items()},
         "cpu_baseline": cpu,
         "adjoint": adj,
