@@ -268,6 +268,7 @@ int hsx_batch_create(int nshots, const hsx_grid *grids, const void *const *coef,
         // n1 faces in one 16-byte pass where they qualify; the rest stay with
         // the generic face kernels (the boundary-plane set zmask / zidx is kept)
         N1Faces &x = n1[s];
+        sets[s].vec1 = n1_groups_ok(&g, A, sets[s]) && n1_groups_ok(&g, B, sets[s]);
         x.sides = n1_vector_sides(&g, A, sets[s], &x.lo, &x.hi);
         if (x.sides) {
             const Face &f = (x.sides & 1) ? x.lo : x.hi;

@@ -63,6 +63,8 @@ struct FaceSet {
     int n;                       // faces in f
     int64_t zidx[3][2];          // boundary plane index per axis / side
     int zmask;                   // faces of the mask
+    int vec1;                    // n2 / n3 faces may use 16-byte groups along n1 (n1 a multiple
+                                 // of the vector width, grid 16-byte aligned)
 };
 // faces of face_mask; closed_form: the two-launch form applies; most / most_plane:
 // largest halo slab and boundary plane (points), for the launch sizes
@@ -75,6 +77,8 @@ int bc_batch_launch(int storage, int nshots, void *const *d_fields, const FaceSe
 // high face, 0 = not applicable) when R is a multiple of the vector width, n1
 // a multiple of it, u 16-byte aligned and the halo groups aligned
 int n1_vector_sides(const hsx_grid *g, const void *u, const FaceSet &fs, Face *lo, Face *hi);
+// whether the n2 / n3 faces of fs may run in 16-byte groups along n1 (FaceSet::vec1)
+int n1_groups_ok(const hsx_grid *g, const void *u, const FaceSet &fs);
 struct N1Faces {
     Face lo, hi;
     int sides;                   // 0: this grid's n1 faces stay with the generic kernels

@@ -144,6 +144,27 @@ def test_free_surface_bit_exact(oracle, capi, dim, n, order, prec):
     _same(d.download(dtype), ref)
 
 
+@pytest.mark.parametrize("n,order,prec", [((21, 11, 13), 2, "f32"), ((21, 12, 10), 6, "f32"), ((22, 11, 12), 6, "f64"),
+                                          ((22, 9, 11), 2, "f64"), ((20, 9, 12), 4, "f32")])
+@pytest.mark.parametrize("mask", [0x3F, 0x3C, 0x0C, 0x30, 0x33])
+def test_free_surface_unaligned_n1_ranges(oracle, capi, n, order, prec, mask):
+    """n1 a multiple of the 16-byte vector but the inner n1 range not aligned to it:
+    the n2 / n3 faces run in 16-byte groups whose edge groups hold points
+    outside the face (left alone), and the n1 faces (where R allows) take the
+    row-end kernel"""
+    dtype, _ = PRECS[prec]
+    p, o = _layouts(oracle, 3, n, order, dtype)
+    vw = 16 // np.dtype(dtype).itemsize
+    print(f"free surface n={n} order={order} prec={prec} mask={mask:#x} n1={p.n1} inner={p.inner()[:2]}", flush=True)
+    assert p.n1 % vw == 0
+    u = _rand(np.random.default_rng(mask + order), p.npoint, dtype)
+    ref = u.copy()
+    oracle.free_surface(o, ref, face_mask=mask)
+    d = capi.DeviceArray.from_numpy(u)
+    capi.free_surface(capi.hsx_grid(p), d, face_mask=mask)
+    _same(d.download(dtype), ref)
+
+
 @pytest.mark.parametrize("n,order", [((7, 9, 6), 8), ((5, 12, 11), 8), ((9, 10, 11), 16), ((4, 4, 4), 2)])
 def test_free_surface_small_grids(oracle, capi, n, order):
     """axes with fewer than R+2 inner points: the six phases run one after the other"""
