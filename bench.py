@@ -506,9 +506,9 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(f"bench.py measures one MI355X; this build runs one process on one GPU "
-              f"(--gpus {args.gpus}, WORLD_SIZE {world} requested). Multi-GPU sharding of the "
-              f"stencil (slabs along n3 with halo exchange) is described in DESIGN.md, not run here.",
+        print(f"bench.py measures one MI355X: one process on one GPU (--gpus {args.gpus}, WORLD_SIZE "
+              f"{world} requested). Sharding the stencil over GPUs (slabs along n3 with one halo "
+              f"exchange per step) is not part of this library.",
               file=sys.stderr)
         sys.exit(2)
 
