@@ -736,7 +736,7 @@ inline int tiled_per_cu(KERN kern, int threads, size_t lds)
 
 // whether a kernel instance keeps more than a few per-lane values in scratch
 // memory (register spills): at 1000^3 FP32 the order-4 64x16 instance spills
-// 36 bytes per lane and is still the fastest shape (3.25 ms against 3.46 ms for
+// 32-36 bytes per lane and is still the fastest shape (3.25 ms against 3.46 ms for
 // 64x8), while spills of 80+ bytes (order 16) cost several times the launch
 constexpr size_t TILED_SPILL_OK = 48;
 template <typename KERN>
