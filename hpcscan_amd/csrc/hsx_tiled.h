@@ -698,8 +698,8 @@ inline int64_t chunk_short(int64_t N3, int R, int64_t tiles)
 // loads before its first output; ties go to the longer chunk. When the tiles
 // fill the device once this is one whole column per tile (1000^3 FP32 order 8:
 // 252 tiles on 256 slots, 3.13 ms against 3.25 ms with 64-plane chunks); 700^3
-// (132 tiles) gets 64-plane chunks in six rounds: 1.12 ms, against 1.62 ms for
-// two chunks per tile (264 workgroups, a second round of 8)
+// (132 tiles) gets 78-plane chunks, 1188 workgroups in five rounds: 1.22 ms,
+// against 1.62 ms for two chunks per tile (264 workgroups: a second round of 8)
 inline int64_t chunk_one_round(int64_t N3, int R, int64_t tiles, int64_t slots)
 {
     const int64_t minc = 4 * R < N3 ? 4 * R : N3;
