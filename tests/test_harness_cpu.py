@@ -214,3 +214,16 @@ def test_c_abi_argument_errors_without_gpu():
     rc = capi.lib().hsx_ac2_step(None, None, None, None, None)
     assert rc != 0
     assert capi.last_error()
+
+
+@pytest.mark.parametrize("argv,env", [(["--gpus", "2"], {}), ([], {"WORLD_SIZE": "2"})])
+def test_bench_refuses_more_than_one_gpu(argv, env):
+    """bench.py measures one GPU: more ranks or --gpus > 1 end with exit code 2
+    and a message, before any device is touched"""
+    import sys
+
+    e = dict(os.environ, **env)
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py")] + argv, capture_output=True, text=True,
+                       timeout=120, env=e)
+    assert r.returncode == 2
+    assert "one MI355X" in r.stderr and r.stdout == ""
