@@ -202,48 +202,80 @@ def gpu_leg(args):
     }
 
 
+def host_cpu():
+    """CPU model, physical cores and logical CPUs of this host (lscpu, /proc/cpuinfo)."""
+    import subprocess
+
+    model, cores, logical = "unknown", None, os.cpu_count()
+    try:
+        info = {}
+        for line in subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+        model = info.get("Model name", model)
+        if info.get("Core(s) per socket") and info.get("Socket(s)"):
+            cores = int(info["Core(s) per socket"]) * int(info["Socket(s)"])
+    except Exception:
+        pass
+    if model == "unknown":
+        try:
+            for line in open("/proc/cpuinfo"):
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
+    return {"model": model, "physical_cores": cores, "logical_cpus": logical}
+
+
+def run_cpu_harness(n, order, nt, ntry, fp64=False, threads=None, timeout=900):
+    """oracle/_build/hpcscan_cpu[_fp64] -testCase Propa -testMode CacheBlk with
+    the headline command line (eigen mode 2,2,2), OMP_PROC_BIND=close and
+    OMP_PLACES=cores; returns (Best Gpoint/s eff., errTestCase, seconds)."""
+    import subprocess
+
+    exe = os.path.join(ROOT, "oracle", "_build", "hpcscan_cpu_fp64" if fp64 else "hpcscan_cpu")
+    if not os.path.exists(exe):
+        raise RuntimeError(f"{exe} is missing: build the oracle (make -C oracle)")
+    env = dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores")
+    if threads:
+        env["OMP_NUM_THREADS"] = str(threads)
+    cmd = [exe, "-testCase", "Propa", "-testMode", "CacheBlk", "-propagator", "Ac2Standard", "-dim", "3",
+           "-n1", str(n), "-n2", str(n), "-n3", str(n), "-fdOrder", str(order), "-nt", str(nt),
+           "-ntry", str(ntry), "-param1", "2", "-param2", "2", "-param3", "2"]
+    t0 = time.perf_counter()
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout, env=env, cwd="/tmp")
+    wall = time.perf_counter() - t0
+    if r.returncode != 0:
+        raise RuntimeError(f"{' '.join(cmd)} failed ({r.returncode}): {r.stdout[-2000:]}{r.stderr[-2000:]}")
+    val = err = None
+    for line in r.stdout.splitlines():
+        t = line.strip()
+        if t.startswith("Best Gpoint/s eff."):
+            val = float(t.split()[-1])
+        elif t.startswith("errTestCase2") or t.startswith("errTestCase"):
+            err = float(t.split()[-1])
+    if val is None:
+        raise RuntimeError("no 'Best Gpoint/s eff.' line in the CPU harness output")
+    return val, err, wall, " ".join(cmd[1:])
+
+
 def cpu_leg(args):
-    """Oracle CacheBlk (cb 9999/4/16, the reference defaults) on the host cores,
-    same grid and step, stopped after about args.cpu_seconds of stepping."""
-    import numpy as np
-
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle as po
-
-    from hpcscan_amd.layout import make_layout
-
-    n, order = args.n, args.order
-    g = po.layout(3, [n, n, n], order)
-    p = make_layout(3, (n, n, n), order)
-    d = [1.0 / (n - 1)] * 3
-    iv = po.inv2(d, np.float32)
-    prn = np.zeros(g.npoint, np.float32)
-    prc = np.zeros(g.npoint, np.float32)
-    coef = np.zeros(g.npoint, np.float32)
-    dt = np.float32(propa_scalars(n, order))
-    po.fill_const(g, coef, po.INNER_POINTS, float(np.float32(dt * dt)))
-    orig = [float(np.float32(-g.ax[a].inner_start * d[a])) for a in range(3)]
-    gamma = math.sqrt(12)
-    po.fill_sep(g, prn, po.ALL_POINTS, (1, 1, 1), [PI_REF * 2] * 3,
-                math.sin(PI_REF * float(np.float32(-2.0 * dt)) * gamma), orig, d, 3)
-    po.fill_sep(g, prc, po.ALL_POINTS, (1, 1, 1), [PI_REF * 2] * 3,
-                math.sin(PI_REF * float(np.float32(-dt)) * gamma), orig, d, 3)
-    del p
-    s = po.sched(1, (9999, 4, 16))
-    steps, t_used = 0, 0.0
-    while (t_used < args.cpu_seconds or steps < 2) and steps < 1000:
-        t0 = time.perf_counter()
-        po.free_surface(g, prc)
-        po.pressure_fd(g, prn, prc, coef, iv, s)
-        t_used += time.perf_counter() - t0
-        prn, prc = prc, prn
-        steps += 1
-    npt = n ** 3
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    log(f"cpu baseline: {steps} steps in {t_used:.1f} s on {cores} threads")
-    return {"value": npt * steps / t_used / 1e9, "unit": "GPoint/s", "cores": cores, "kind": "port",
-            "sample": f"{steps} steps of free surface + fused Ac2 step, {n}^3 FP32 O{order}, "
-                      f"oracle CacheBlk cb 9999/4/16, {cores} OpenMP threads"}
+    """The reference's CPU path restated (oracle/_build/hpcscan_cpu, testMode
+    CacheBlk, cb 9999/4/16 as src/config.cpp:34-36) on this host's cores: the
+    headline command line with a bounded number of steps (args.cpu_nt, one
+    try).  value = its "Best Gpoint/s eff." report line."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or None
+    hc = host_cpu()
+    val, err, wall, cmd = run_cpu_harness(args.n, args.order, args.cpu_nt, 1, threads=threads)
+    cores = threads or os.cpu_count() or 1
+    log(f"cpu baseline: {val:.3f} GPoint/s ({args.cpu_nt} steps, {wall:.1f} s run) on {cores} threads, "
+        f"{hc['model']}")
+    return {"value": val, "unit": "GPoint/s", "cores": cores, "kind": "port",
+            "sample": f"hpcscan_cpu {cmd} (CacheBlk restatement, one try of {args.cpu_nt} steps of the "
+                      f"{args.n}^3 workload), {cores} OpenMP threads, OMP_PROC_BIND=close, OMP_PLACES=cores",
+            "cpu_model": hc["model"], "physical_cores": hc["physical_cores"], "logical_cpus": hc["logical_cpus"],
+            "errTestCase": err}
 
 
 def adjoint_leg(args):
@@ -254,7 +286,9 @@ def adjoint_leg(args):
       ours   hsx_gradient, checkpoints every K steps in the chosen 16-bit format
       plain  the same scheme in PyTorch tensor ops, FP32, autograd (full storage)
     Errors: relative L2 against the FP64 gradient (hsx_gradient on FP64 grids
-    with storage-type checkpoints, i.e. exact FP64 recomputation)."""
+    with storage-type checkpoints, i.e. exact FP64 recomputation).  Gradients
+    run from a plan (work buffers allocated once per geometry, outside the
+    timing); the *_allocating figures allocate inside the call."""
     import numpy as np
     import torch
 
@@ -320,16 +354,25 @@ def adjoint_leg(args):
     with_source(S32, srcs[0])
     d64, dobs64, dobs32 = observed(S64)
 
-    def grad_of(S, dobs, every, f, reps=1):
+    def grad_of(S, dobs, every, f, reps=1, alloc_each=False):
+        """best-of-reps seconds of one gradient; with a plan made beforehand
+        (the work buffers allocated once per geometry) unless alloc_each, where
+        each call allocates and frees them (hsx_gradient)"""
+        plan = None if alloc_each else capi.AdjointPlan(S["g"], nt, len(rec), every, f)
         best, gout, J = 1e30, None, None
         for _ in range(reps):
             gr = capi.DeviceArray.from_numpy(np.zeros(p.npoint))
             capi.sync()
             t0 = time.perf_counter()
-            J = capi.gradient(S["g"], S["coef"], S["shot"], dobs, gr, every, f)
+            if plan is None:
+                J = capi.gradient(S["g"], S["coef"], S["shot"], dobs, gr, every, f)
+            else:
+                J = plan.gradient(S["g"], S["coef"], S["shot"], dobs, gr)
             best = min(best, time.perf_counter() - t0)
             gout = gr.download(np.float64)
             gr.free()
+        if plan is not None:
+            plan.close()
         return best, gout, J
 
     t64, g64, J64 = grad_of(S64, d64, args.adj_fp64_every or K, capi.HSX_CKPT_FP32)
@@ -350,12 +393,17 @@ def adjoint_leg(args):
         t64 /= nshots
     nrm = np.linalg.norm(g64)
     err_ours = float(np.linalg.norm(g_ours - g64) / nrm)
-    full_t = err_full = None
+    full_t = err_full = full_alloc_t = ours_alloc_t = None
+    if nshots == 1:
+        ours_alloc_t, _, _ = grad_of(S32, dobs32, K, fmt, alloc_each=True)
     if nshots == 1 and (nt + 2) * p.npoint * 4 < 150e9:      # every step kept: only where it fits in HBM
         full_t, g_full, _ = grad_of(S32, dobs32, 0, capi.HSX_CKPT_FP32, reps=2)
         err_full = float(np.linalg.norm(g_full - g64) / nrm)
-    full_txt = "skipped" if full_t is None else f"{full_t:.3f} s (err {err_full:.2e})"
-    log(f"adjoint: ours {ours_t:.3f} s (err {err_ours:.2e}), full storage FP32 {full_txt}, FP64 {t64:.3f} s")
+        full_alloc_t, _, _ = grad_of(S32, dobs32, 0, capi.HSX_CKPT_FP32, alloc_each=True)
+    full_txt = "skipped" if full_t is None else f"{full_t:.3f} s (err {err_full:.2e}; {full_alloc_t:.3f} s allocating)"
+    alloc_txt = "" if ours_alloc_t is None else f"; {ours_alloc_t:.3f} s allocating"
+    log(f"adjoint: ours {ours_t:.3f} s (err {err_ours:.2e}{alloc_txt}), full storage FP32 {full_txt}, "
+        f"FP64 {t64:.3f} s")
 
     # plain PyTorch, FP32, autograd over the whole time loop
     plain_t, err_plain = None, None
@@ -380,6 +428,7 @@ def adjoint_leg(args):
                     f", {n}^3 FP32 FD order {order}, {nt} steps, h 10 m, smooth velocity "
                     f"2000-4000 m/s, 8 Hz Ricker top centre, {len(rec)} receivers, data from a +-3% perturbed model",
         "s_per_shot": round(ours_t, 4),
+        "s_per_shot_allocating": None if ours_alloc_t is None else round(ours_alloc_t, 4),
         "checkpoints": f"every {K} steps, {args.adj_format} with per-snapshot max-abs scale, FP64 gradient",
         "grad_rel_l2_vs_fp64": err_ours,
         "plain_s_per_shot": None if plain_t is None else round(plain_t, 4),
@@ -387,6 +436,8 @@ def adjoint_leg(args):
         "plain_grad_rel_l2_vs_fp64": err_plain,
         "vs_plain": None if plain_t is None else round(plain_t / ours_t, 3),
         "full_storage_fp32_s_per_shot": None if full_t is None else round(full_t, 4),
+        "full_storage_fp32_s_per_shot_allocating": None if full_alloc_t is None else round(full_alloc_t, 4),
+        "vs_full_storage": None if full_t is None else round(full_t / ours_t, 3),
         "full_storage_fp32_grad_rel_l2_vs_fp64": err_full,
         "fp64_reference_s": round(t64, 4),
         "misfit_fp32": J32, "misfit_fp64": J64,
@@ -456,6 +507,7 @@ def batch_leg(args):
         "workload": f"{ns} forward shots, N^3 with N in [{min(sizes)}, {max(sizes)}], FP32 order 8, {nt} steps, "
                     f"layered 1500-4500 m/s x (1 + 0.05 G), 10 Hz Ricker",
         "gpoint_per_s": round(pts * nt / t_graph / 1e9, 3),
+        "roofline_frac": round(16 * pts * nt / t_graph / 1e9 / HBM_PEAK_GBS, 4),
         "s_graph": round(t_graph, 4), "s_no_graph": round(t_nograph, 4), "s_one_by_one": round(t_single, 4),
         "traces_equal_single_shot": bool(same),
     }
@@ -469,7 +521,8 @@ def main():
     ap.add_argument("--n", type=int, default=1000, help="inner points per axis")
     ap.add_argument("--order", type=int, default=8, choices=[4, 8, 12, 16])
     ap.add_argument("--kernel", default="auto", choices=["auto", "ref", "tiled"])
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-nt", type=int, default=40,
+                    help="time steps of the bounded CPU-baseline sample (about 10-20 s at 1000^3 on 16 cores)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile-lx", type=int, default=0, choices=[0, 16, 32, 64],
                     help="lanes along n1 per tile of the fused kernel (0 = chosen from the grid width)")
@@ -485,7 +538,8 @@ def main():
                     help="load each plane's halo one plane ahead instead of with the queue front")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one full-grid copy kernel first (counter calibration for profiles)")
-    ap.add_argument("--batch", action="store_true", help="add the batched-shots leg (32 uneven shots)")
+    ap.add_argument("--no-batch", dest="batch", action="store_false",
+                    help="skip the batched-shots leg (32 uneven shots)")
     ap.add_argument("--batch-shots", type=int, default=32)
     ap.add_argument("--batch-nt", type=int, default=1000)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient leg")

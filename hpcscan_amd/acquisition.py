@@ -45,19 +45,39 @@ def receiver_line(p, count=None, depth=1):
     return np.array([flat_index(p, r[0] + depth, j, i3) for j in i2], np.int64)
 
 
+def _linear_resample_matrix(n_in, n_out):
+    """(n_out, n_in) weights of linear interpolation with the end points of both
+    grids aligned (output point o sits at input coordinate o (n_in-1)/(n_out-1))"""
+    x = np.arange(n_out) * ((n_in - 1) / max(1, n_out - 1))
+    i0 = np.minimum(np.floor(x).astype(np.int64), n_in - 1)
+    i1 = np.minimum(i0 + 1, n_in - 1)
+    w = x - i0
+    m = np.zeros((n_out, n_in))
+    m[np.arange(n_out), i0] += 1.0 - w
+    m[np.arange(n_out), i1] += w
+    return m
+
+
 def gaussian_field(shape, seed, corr_points=12.0):
     """smooth Gaussian random field of unit standard deviation: white noise on a
-    coarse grid, Gaussian-filtered, trilinearly resampled to `shape`"""
+    coarse grid, Gaussian-filtered, trilinearly resampled to `shape` (one axis
+    at a time, as three small matrix products)"""
     from scipy import ndimage
 
     rng = np.random.default_rng(seed)
     step = max(2, int(corr_points // 3))
     coarse = tuple(max(4, s // step + 2) for s in shape)
     f = ndimage.gaussian_filter(rng.standard_normal(coarse), sigma=corr_points / step, mode="wrap")
-    f = ndimage.zoom(f, [s / c for s, c in zip(shape, coarse)], order=1)[:shape[0], :shape[1], :shape[2]]
-    f = f - f.mean()
-    sd = f.std()
-    return f / sd if sd > 0 else f
+    m0, m1, m2 = (_linear_resample_matrix(c, s) for c, s in zip(coarse, shape))
+    f = np.tensordot(m0, f, axes=(1, 0))                  # (n3, c2, c1)
+    f = np.tensordot(m1, f, axes=(1, 1))                  # (n2, n3, c1)
+    f = np.ascontiguousarray(f.transpose(1, 0, 2)) @ m2.T  # (n3, n2, n1)
+    flat = f.reshape(-1)
+    flat -= flat.sum() / flat.size
+    sd = math.sqrt(float(np.dot(flat, flat)) / flat.size)
+    if sd > 0:
+        flat /= sd
+    return f
 
 
 def layered_velocity(p, vmin, vmax, sigma=0.05, seed=0):
@@ -67,10 +87,14 @@ def layered_velocity(p, vmin, vmax, sigma=0.05, seed=0):
     i1 = np.arange(p.n1)
     depth = np.clip((i1 - r[0]) / max(1, r[1] - r[0]), 0.0, 1.0)
     v1 = vmin + (vmax - vmin) * depth
-    v = np.broadcast_to(v1[None, None, :], (p.n3, p.n2, p.n1)).astype(np.float64)
-    if sigma > 0:
-        v = v * (1.0 + sigma * gaussian_field((p.n3, p.n2, p.n1), seed))
-    return np.clip(v, vmin, vmax).ravel()
+    if sigma <= 0:
+        return np.broadcast_to(v1[None, None, :], (p.n3, p.n2, p.n1)).astype(np.float64).ravel()
+    v = gaussian_field((p.n3, p.n2, p.n1), seed)    # updated in place: no grid-sized temporaries
+    v *= sigma
+    v += 1.0
+    v *= v1[None, None, :]
+    np.clip(v, vmin, vmax, out=v)
+    return v.ravel()
 
 
 def smooth_velocity(p, vmin, vmax, seed=0):

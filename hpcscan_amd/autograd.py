@@ -1,6 +1,8 @@
 """torch.autograd wrapper of shot modeling, for tests: the forward is
 hsx_forward, the vector-Jacobian product is hsx_gradient (the hand-written
-adjoint) with the incoming trace gradient as residual.
+adjoint) with the incoming trace gradient passed as the residual itself
+(HSX_ADJ_RESIDUAL), so that a loss of any scale or weighting reaches it
+unrounded.
 
 Device arrays are torch tensors on the GPU; the library is called through the
 C ABI (hpcscan_amd.capi), which fails loudly when it is not built.
@@ -27,7 +29,7 @@ class Ac2Shot:
         self.nt = int(self.wavelet.shape[0])
         self.nrec = int(self.rec.numel())
         self.desc = capi.make_shot(self.nt, self.dt, (self.src, int(self.src.numel())), self.wavelet,
-                                   (self.rec, self.nrec))
+                                   (self.rec, self.nrec), npoint=layout.npoint, host_src=src_idx, host_rec=rec_idx)
         r = layout.inner()
         mask = torch.zeros((layout.n3, layout.n2, layout.n1), dtype=torch.bool, device=device)
         mask[r[4]:r[5] + 1, r[2]:r[3] + 1, r[0]:r[1] + 1] = True
@@ -57,11 +59,11 @@ class Ac2Modeling(torch.autograd.Function):
     def backward(ctx, grad_traces):
         coef, traces = ctx.saved_tensors
         shot = ctx.shot
-        # hsx_gradient uses residual = traces - d_obs: pass d_obs = traces - grad
-        d_obs = (traces - grad_traces.to(shot.dtype)).contiguous()
+        # the incoming trace gradient is the adjoint source itself (HSX_ADJ_RESIDUAL)
+        res = grad_traces.to(shot.dtype).contiguous()
         grad = torch.zeros(coef.numel(), dtype=torch.float64, device=coef.device)
         torch.cuda.synchronize(coef.device)
-        capi.gradient(shot.g, coef, shot.desc, d_obs, grad, ctx.every, ctx.fmt)
+        capi.gradient(shot.g, coef, shot.desc, res, grad, ctx.every, ctx.fmt, residual=True)
         return grad.to(ctx.m_dtype), None, None, None
 
 

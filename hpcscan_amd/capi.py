@@ -105,6 +105,9 @@ SIGNATURES = {
     "hsx_ckpt_decode": (_i, [_vp, _i64, _i, _vp, _vp, _vp]),
     "hsx_forward": (_i, [_G, _vp, _vp, _vp, _vp]),
     "hsx_gradient": (_i, [_G, _vp, _vp, _vp, _vp, _vp, _P(_d), _vp]),
+    "hsx_adjoint_plan_create": (_i, [_G, _i64, _i64, _vp, _P(_vp)]),
+    "hsx_gradient_plan": (_i, [_vp, _G, _vp, _vp, _vp, _vp, _P(_d), _vp]),
+    "hsx_adjoint_plan_destroy": (_i, [_vp]),
     "hsx_autotune": (_i, [_G, _i, _P(_i), _P(ctypes.c_float)]),
     "hsx_autotune_clear": (_i, []),
     "hsx_batch_create": (_i, [_i, _G, _P(_vp), _vp, _P(_vp), _P(_vp)]),
@@ -120,7 +123,10 @@ class HsxShot(ctypes.Structure):
 
 
 class HsxAdjointOpts(ctypes.Structure):
-    _fields_ = [("ckpt_every", ctypes.c_int), ("ckpt_format", ctypes.c_int)]
+    _fields_ = [("ckpt_every", ctypes.c_int), ("ckpt_format", ctypes.c_int), ("flags", ctypes.c_int)]
+
+
+HSX_ADJ_RESIDUAL = 1
 
 
 HSX_INJECT_ADD, HSX_INJECT_SET = 0, 1
@@ -374,9 +380,25 @@ def ckpt_decode(packed, n, fmt, base, out, stream=None):
     check(lib().hsx_ckpt_decode(dptr(packed), int(n), fmt, dptr(base), dptr(out), stream))
 
 
-def make_shot(nt, dt, src_idx, wavelet, rec_idx) -> HsxShot:
+def check_indices(idx, npoint: int, what: str = "index"):
+    """Raise ValueError unless every flat grid index lies in [0, npoint)."""
+    a = np.asarray(idx, dtype=np.int64).reshape(-1)
+    if a.size and (a.min() < 0 or a.max() >= npoint):
+        bad = a[(a < 0) | (a >= npoint)][0]
+        raise ValueError(f"{what} {int(bad)} outside the grid's {npoint} points")
+
+
+def make_shot(nt, dt, src_idx, wavelet, rec_idx, npoint: Optional[int] = None, host_src=None,
+              host_rec=None) -> HsxShot:
     """Shot descriptor over DEVICE arrays (DeviceArray / tensors / addresses);
-    the caller keeps them alive."""
+    the caller keeps them alive.  With npoint, the host copies of the indices
+    (host_src / host_rec, or the tensors themselves) are checked first."""
+    if npoint is not None:
+        for what, dev, host in (("source index", src_idx, host_src), ("receiver index", rec_idx, host_rec)):
+            if host is None and isinstance(dev, tuple) and torch is not None and isinstance(dev[0], torch.Tensor):
+                host = dev[0].detach().cpu().numpy()
+            if host is not None:
+                check_indices(host, npoint, what)
     s = HsxShot()
     s.nt, s.dt = int(nt), float(dt)
     s.nsrc = int(src_idx[1]) if isinstance(src_idx, tuple) else 0
@@ -421,14 +443,50 @@ def forward(g: HsxGrid, coef, shot: HsxShot, traces, stream=None):
     check(lib().hsx_forward(ctypes.byref(g), dptr(coef), ctypes.byref(shot), dptr(traces), stream))
 
 
-def gradient(g: HsxGrid, coef, shot: HsxShot, d_obs, grad, ckpt_every=0, ckpt_format=HSX_CKPT_FP32,
-             stream=None) -> float:
+def _opts(ckpt_every, ckpt_format, residual):
     o = HsxAdjointOpts()
     o.ckpt_every, o.ckpt_format = int(ckpt_every), int(ckpt_format)
+    o.flags = HSX_ADJ_RESIDUAL if residual else 0
+    return o
+
+
+def gradient(g: HsxGrid, coef, shot: HsxShot, d_obs, grad, ckpt_every=0, ckpt_format=HSX_CKPT_FP32,
+             stream=None, residual=False) -> float:
+    """hsx_gradient; residual=True: d_obs is the residual (HSX_ADJ_RESIDUAL)."""
+    o = _opts(ckpt_every, ckpt_format, residual)
     mis = ctypes.c_double(0.0)
     check(lib().hsx_gradient(ctypes.byref(g), dptr(coef), ctypes.byref(shot), dptr(d_obs), ctypes.byref(o),
                              dptr(grad), ctypes.byref(mis), stream))
     return mis.value
+
+
+class AdjointPlan:
+    """hsx_adjoint_plan: the gradient's work buffers allocated once for one
+    grid layout, nt and receiver count."""
+
+    def __init__(self, g: HsxGrid, nt, nrec, ckpt_every=0, ckpt_format=HSX_CKPT_FP32, residual=False):
+        self._o = _opts(ckpt_every, ckpt_format, residual)
+        h = ctypes.c_void_p()
+        check(lib().hsx_adjoint_plan_create(ctypes.byref(g), int(nt), int(nrec), ctypes.byref(self._o),
+                                            ctypes.byref(h)))
+        self.h = h
+
+    def gradient(self, g: HsxGrid, coef, shot: HsxShot, d_obs, grad, stream=None) -> float:
+        mis = ctypes.c_double(0.0)
+        check(lib().hsx_gradient_plan(self.h, ctypes.byref(g), dptr(coef), ctypes.byref(shot), dptr(d_obs),
+                                      dptr(grad), ctypes.byref(mis), stream))
+        return mis.value
+
+    def close(self):
+        if self.h is not None and self.h.value:
+            lib().hsx_adjoint_plan_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Event:

@@ -21,13 +21,14 @@
 
 namespace hsx {
 
-// res = d - dobs; partial[block] = sum of res^2 over the block (double)
+// res = d - dobs (d NULL: res = dobs, the residual given by the caller);
+// partial[block] = sum of res^2 over the block (double)
 template <typename TS>
 __global__ void __launch_bounds__(256) k_residual(const TS *d, const TS *dobs, TS *res, int64_t n, double *partial)
 {
     double acc = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const TS r = d[i] - dobs[i];
+        const TS r = d ? d[i] - dobs[i] : dobs[i];
         res[i] = r;
         acc += (double)r * (double)r;
     }
@@ -47,12 +48,18 @@ __global__ void k_sum_partials(const double *partial, int n, double *out)
     }
 }
 
-// grad[src_i] += scale * mu[src_i] * f_i (the source term of the imaging condition)
+// grad[src_i] += scale * mu[src_i] * f_i (the source term of the imaging
+// condition), for the sources inside the imaging box b only
 template <typename TS>
-__global__ void k_imaging_src(double *grad, const TS *mu, int64_t nsrc, const int64_t *idx, const TS *f, double scale)
+__global__ void k_imaging_src(double *grad, const TS *mu, int64_t nsrc, const int64_t *idx, const TS *f, double scale,
+                              Box b, int64_t n1, int64_t n12)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nsrc) grad[idx[i]] += scale * (double)mu[idx[i]] * (double)f[i];
+    if (i >= nsrc) return;
+    const int64_t j = idx[i];
+    const int64_t i3 = j / n12, i2 = (j - i3 * n12) / n1, i1 = j - i3 * n12 - i2 * n1;
+    if (i1 < b.s1 || i1 > b.e1 || i2 < b.s2 || i2 > b.e2 || i3 < b.s3 || i3 > b.e3) return;
+    grad[j] += scale * (double)mu[j] * (double)f[i];
 }
 
 struct Dev {
@@ -104,6 +111,175 @@ static int fwd_step(const hsx_grid *g, const void *coef, const hsx_shot *sh, voi
 
 } // namespace hsx
 
+// Work buffers of hsx_gradient for one grid layout, nt, receiver count and
+// checkpoint options, allocated once (hsx_adjoint_plan_create) and reused by
+// every gradient of that geometry.
+struct hsx_adjoint_plan {
+    hsx_grid g;
+    int64_t nt = 0, nrec = 0;
+    hsx_adjoint_opts opt;
+    size_t es = 4, bytes = 0, snap = 0;
+    int64_t nseg = 0;
+    hsx::Dev traces, res, crec, partial, msum, m1, m2, ck, fa, fb;
+    std::vector<hsx::Dev> q;   // wavefields q^{m}, m from the segment start - 1
+};
+
+namespace hsx {
+
+static int plan_checks(const hsx_grid *g, int64_t nt, int64_t nrec, const hsx_adjoint_opts *opt)
+{
+    int rc = check_grid(g);
+    if (rc) return rc;
+    if ((rc = check_ptr(opt, "opts"))) return rc;
+    if (nt < 1) return fail(HSX_ERR_INVALID, "nt = %lld", (long long)nt);
+    if (nrec < 0) return fail(HSX_ERR_INVALID, "negative receiver count");
+    const int K = opt->ckpt_every;
+    const int fmt = opt->ckpt_format;
+    if (K < 0) return fail(HSX_ERR_INVALID, "ckpt_every %d < 0", K);
+    if (fmt != HSX_CKPT_FP32 && fmt != HSX_CKPT_FP16 && fmt != HSX_CKPT_BF16)
+        return fail(HSX_ERR_INVALID, "checkpoint format %d", fmt);
+    if (fmt != HSX_CKPT_FP32 && g->storage != HSX_FP32)
+        return fail(HSX_ERR_UNSUPPORTED, "16-bit checkpoints of FP64 wavefields");
+    if (K == 0 && fmt != HSX_CKPT_FP32) return fail(HSX_ERR_INVALID, "full storage keeps every step in the storage type");
+    if (opt->flags & ~HSX_ADJ_RESIDUAL) return fail(HSX_ERR_INVALID, "adjoint flags 0x%x", opt->flags);
+    for (int a = 0; a < g->dim; a++)
+        if (g->inner[2 * a + 1] - g->inner[2 * a] < 2) return fail(HSX_ERR_INVALID, "grid too small for the imaging range");
+    return ensure_init();
+}
+
+static bool same_layout(const hsx_grid &a, const hsx_grid &b)
+{
+    if (a.dim != b.dim || a.fd_order != b.fd_order || a.storage != b.storage || a.compute != b.compute ||
+        a.n1 != b.n1 || a.n2 != b.n2 || a.n3 != b.n3)
+        return false;
+    for (int k = 0; k < 6; k++)
+        if (a.inner[k] != b.inner[k]) return false;
+    return true;
+}
+
+static int plan_run(hsx_adjoint_plan *P, const hsx_grid *g, const void *coef, const hsx_shot *sh, const void *d_obs,
+                    double *grad, double *misfit, hipStream_t s)
+{
+    int rc;
+    const int K = P->opt.ckpt_every;
+    const int fmt = P->opt.ckpt_format;
+    const bool given_residual = (P->opt.flags & HSX_ADJ_RESIDUAL) != 0;
+    const size_t es = P->es, bytes = P->bytes, snap = P->snap;
+    const int64_t npoint = g->n1 * g->n2 * g->n3;
+    const int64_t nt = sh->nt, nrec = sh->nrec, ntr = nt * nrec, nseg = P->nseg;
+
+    // imaging range: the inner points minus the boundary planes zeroed by B
+    int64_t rim[6];
+    for (int a = 0; a < 3; a++) {
+        rim[2 * a] = g->inner[2 * a] + (a < g->dim ? 1 : 0);
+        rim[2 * a + 1] = g->inner[2 * a + 1] - (a < g->dim ? 1 : 0);
+    }
+    const Box rimb = box_of(rim);
+    const double img_scale = -1.0 / (sh->dt * sh->dt);
+    auto &q = P->q;
+
+    auto save = [&](int64_t seg, const void *cur, const void *prev) -> int {
+        char *dst = P->ck.c() + (size_t)seg * 2 * snap;
+        if (fmt == HSX_CKPT_FP32) {
+            HSX_HIP(hipMemcpyAsync(dst, cur, bytes, hipMemcpyDeviceToDevice, s));
+            HSX_HIP(hipMemcpyAsync(dst + snap, prev, bytes, hipMemcpyDeviceToDevice, s));
+            return HSX_OK;
+        }
+        int r = hsx_ckpt_encode((const float *)cur, nullptr, npoint, dst, fmt, s);
+        if (!r) r = hsx_ckpt_encode((const float *)cur, (const float *)prev, npoint, dst + snap, fmt, s);
+        return r;
+    };
+    auto restore = [&](int64_t seg, void *cur, void *prev) -> int {
+        const char *src = P->ck.c() + (size_t)seg * 2 * snap;
+        if (fmt == HSX_CKPT_FP32) {
+            HSX_HIP(hipMemcpyAsync(cur, src, bytes, hipMemcpyDeviceToDevice, s));
+            HSX_HIP(hipMemcpyAsync(prev, src + snap, bytes, hipMemcpyDeviceToDevice, s));
+            return HSX_OK;
+        }
+        int r = hsx_ckpt_decode(src, npoint, fmt, nullptr, (float *)cur, s);
+        if (!r) r = hsx_ckpt_decode(src + snap, npoint, fmt, (const float *)cur, (float *)prev, s);
+        return r;
+    };
+
+    // ---- forward pass (traces only when the residual is formed here)
+    void *tr = given_residual ? nullptr : P->traces.p;
+    if (K == 0) {
+        if ((rc = zero(g, q[0].p, s)) || (rc = zero(g, q[1].p, s))) return rc;
+        for (int64_t n = 0; n < nt; n++)
+            if ((rc = fwd_step(g, coef, sh, q[n + 2].p, q[n].p, q[n + 1].p, n, tr, es, s))) return rc;
+    } else {
+        if ((rc = zero(g, P->fa.p, s)) || (rc = zero(g, P->fb.p, s))) return rc;
+        void *prev = P->fa.p, *cur = P->fb.p;
+        for (int64_t n = 0; n < nt; n++) {
+            if (n % K == 0 && (rc = save(n / K, cur, prev))) return rc;
+            if ((rc = fwd_step(g, coef, sh, prev, prev, cur, n, tr, es, s))) return rc;
+            std::swap(prev, cur);
+        }
+    }
+
+    // ---- residual and misfit
+    const int nb = (int)std::max<int64_t>(1, std::min<int64_t>((ntr + 255) / 256, 4096));
+    rc = dispatch_types(g, [&](auto ts, auto) -> int {
+        using TS = decltype(ts);
+        hipLaunchKernelGGL(k_residual<TS>, dim3(nb), dim3(256), 0, s, (const TS *)tr, (const TS *)d_obs,
+                           (TS *)P->res.p, ntr, (double *)P->partial.p);
+        HSX_LAUNCHED("k_residual");
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, s, (const double *)P->partial.p, nb,
+                           (double *)P->msum.p);
+        HSX_LAUNCHED("k_sum_partials");
+        return HSX_OK;
+    });
+    if (rc) return rc;
+    if (nrec > 0 && (rc = hsx_gather(g, coef, nrec, sh->rec_idx, P->crec.p, s))) return rc;
+
+    // ---- backward pass: mu^k for k = nt .. 1
+    if ((rc = zero(g, P->m1.p, s)) || (rc = zero(g, P->m2.p, s))) return rc;
+    void *mu1 = P->m1.p, *mu2 = P->m2.p;   // mu^{k+1}, mu^{k+2}
+    auto back_step = [&](int64_t k, const void *qk, const void *qk1, const void *qk2) -> int {
+        int r = hsx_free_surface(g, mu1, 0x3F, s);
+        if (!r) r = hsx_ac2_step(g, mu2, mu1, coef, s);                    // mu2 <- mu^k
+        if (!r && nrec > 0)
+            r = hsx_inject(g, mu2, nrec, sh->rec_idx, P->res.c() + (size_t)(k - 1) * nrec * es, P->crec.p, 1.0,
+                           HSX_INJECT_ADD, s);
+        if (!r) r = hsx_imaging(g, grad, mu2, qk, qk1, qk2, rim, img_scale, -1, 0.0, s);
+        if (!r && sh->nsrc > 0) {
+            r = dispatch_types(g, [&](auto ts, auto) -> int {
+                using TS = decltype(ts);
+                hipLaunchKernelGGL(k_imaging_src<TS>, dim3((unsigned)((sh->nsrc + 63) / 64)), dim3(64), 0, s, grad,
+                                   (const TS *)mu2, sh->nsrc, sh->src_idx,
+                                   (const TS *)((const char *)sh->wavelet + (size_t)(k - 1) * sh->nsrc * es),
+                                   -img_scale, rimb, g->n1, g->n1 * g->n2);
+                HSX_LAUNCHED("k_imaging_src");
+                return HSX_OK;
+            });
+        }
+        std::swap(mu1, mu2);
+        return r;
+    };
+    if (K == 0) {
+        for (int64_t k = nt; k >= 1; k--)
+            if ((rc = back_step(k, q[k + 1].p, q[k].p, q[k - 1].p))) return rc;
+    } else {
+        for (int64_t seg = nseg - 1; seg >= 0; seg--) {
+            const int64_t n0 = seg * K, n1 = std::min<int64_t>(n0 + K, nt);
+            // q[j] holds q^{n0 - 1 + j}
+            if ((rc = restore(seg, q[1].p, q[0].p))) return rc;
+            for (int64_t n = n0; n < n1; n++)
+                if ((rc = fwd_step(g, coef, sh, q[n - n0 + 2].p, q[n - n0].p, q[n - n0 + 1].p, n, nullptr, es, s)))
+                    return rc;
+            for (int64_t k = n1; k > n0; k--)
+                if ((rc = back_step(k, q[k - n0 + 1].p, q[k - n0].p, q[k - n0 - 1].p))) return rc;
+        }
+    }
+    double m = 0.0;
+    HSX_HIP(hipMemcpyAsync(&m, P->msum.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    HSX_HIP(hipStreamSynchronize(s));
+    if (misfit) *misfit = 0.5 * m;
+    return HSX_OK;
+}
+
+} // namespace hsx
+
 using namespace hsx;
 
 extern "C" {
@@ -128,6 +304,65 @@ int hsx_forward(const hsx_grid *g, const void *coef, const hsx_shot *sh, void *t
     return HSX_OK;
 }
 
+int hsx_adjoint_plan_create(const hsx_grid *g, int64_t nt, int64_t nrec, const hsx_adjoint_opts *opt,
+                            hsx_adjoint_plan **out)
+{
+    if (!out) return fail(HSX_ERR_INVALID, "null plan pointer");
+    *out = nullptr;
+    int rc = plan_checks(g, nt, nrec, opt);
+    if (rc) return rc;
+    hsx_adjoint_plan *P = new hsx_adjoint_plan();
+    P->g = *g;
+    P->nt = nt;
+    P->nrec = nrec;
+    P->opt = *opt;
+    P->es = g->storage == HSX_FP64 ? 8 : 4;
+    const int64_t npoint = g->n1 * g->n2 * g->n3;
+    P->bytes = (size_t)npoint * P->es;
+    const int K = opt->ckpt_every;
+    const int64_t ntr = nt * nrec;
+    P->nseg = K == 0 ? 0 : (nt + K - 1) / K;
+    P->snap = opt->ckpt_format == HSX_CKPT_FP32 ? P->bytes : hsx_ckpt_bytes(npoint);
+    auto bail = [&](int code) {
+        delete P;
+        return code;
+    };
+    if ((rc = P->traces.alloc(ntr * P->es)) || (rc = P->res.alloc(ntr * P->es)) || (rc = P->crec.alloc(nrec * P->es)) ||
+        (rc = P->partial.alloc(4096 * sizeof(double))) || (rc = P->msum.alloc(sizeof(double))) ||
+        (rc = P->m1.alloc(P->bytes)) || (rc = P->m2.alloc(P->bytes)))
+        return bail(rc);
+    P->q.resize(K == 0 ? nt + 2 : K + 2);
+    for (auto &x : P->q)
+        if ((rc = x.alloc(P->bytes))) return bail(rc);
+    if (K > 0 && ((rc = P->ck.alloc((size_t)P->nseg * 2 * P->snap)) || (rc = P->fa.alloc(P->bytes)) ||
+                  (rc = P->fb.alloc(P->bytes))))
+        return bail(rc);
+    *out = P;
+    return HSX_OK;
+}
+
+int hsx_adjoint_plan_destroy(hsx_adjoint_plan *P)
+{
+    if (!P) return HSX_OK;
+    (void)hipDeviceSynchronize();   // no kernel of a gradient may still use the buffers
+    delete P;
+    return HSX_OK;
+}
+
+int hsx_gradient_plan(hsx_adjoint_plan *P, const hsx_grid *g, const void *coef, const hsx_shot *sh, const void *d_obs,
+                      double *grad, double *misfit, void *stream)
+{
+    if (!P) return fail(HSX_ERR_INVALID, "null plan");
+    int rc = shot_checks(g, coef, sh);
+    if (rc) return rc;
+    if ((rc = check_ptr(d_obs, "d_obs")) || (rc = check_ptr(grad, "grad"))) return rc;
+    if (!same_layout(*g, P->g)) return fail(HSX_ERR_INVALID, "gradient: grid layout differs from the plan's");
+    if (sh->nt != P->nt || sh->nrec != P->nrec)
+        return fail(HSX_ERR_INVALID, "gradient: nt / receiver count %lld / %lld, plan %lld / %lld", (long long)sh->nt,
+                    (long long)sh->nrec, (long long)P->nt, (long long)P->nrec);
+    return plan_run(P, g, coef, sh, d_obs, grad, misfit, stream_of(stream));
+}
+
 int hsx_gradient(const hsx_grid *g, const void *coef, const hsx_shot *sh, const void *d_obs,
                  const hsx_adjoint_opts *opt, double *grad, double *misfit, void *stream)
 {
@@ -135,142 +370,11 @@ int hsx_gradient(const hsx_grid *g, const void *coef, const hsx_shot *sh, const 
     if (rc) return rc;
     if ((rc = check_ptr(d_obs, "d_obs")) || (rc = check_ptr(grad, "grad")) || (rc = check_ptr(opt, "opts")))
         return rc;
-    const int K = opt->ckpt_every;
-    const int fmt = opt->ckpt_format;
-    if (K < 0) return fail(HSX_ERR_INVALID, "ckpt_every %d < 0", K);
-    if (fmt != HSX_CKPT_FP32 && fmt != HSX_CKPT_FP16 && fmt != HSX_CKPT_BF16)
-        return fail(HSX_ERR_INVALID, "checkpoint format %d", fmt);
-    if (fmt != HSX_CKPT_FP32 && g->storage != HSX_FP32)
-        return fail(HSX_ERR_UNSUPPORTED, "16-bit checkpoints of FP64 wavefields");
-    if (K == 0 && fmt != HSX_CKPT_FP32) return fail(HSX_ERR_INVALID, "full storage keeps every step in the storage type");
-
-    const size_t es = g->storage == HSX_FP64 ? 8 : 4;
-    const int64_t npoint = g->n1 * g->n2 * g->n3;
-    const size_t bytes = (size_t)npoint * es;
-    const int64_t nt = sh->nt, nrec = sh->nrec, ntr = nt * nrec;
-    hipStream_t s = stream_of(stream);
-
-    // imaging range: the inner points minus the boundary planes zeroed by B
-    int64_t rim[6];
-    for (int a = 0; a < 3; a++) {
-        rim[2 * a] = g->inner[2 * a] + (a < g->dim ? 1 : 0);
-        rim[2 * a + 1] = g->inner[2 * a + 1] - (a < g->dim ? 1 : 0);
-        if (rim[2 * a + 1] < rim[2 * a]) return fail(HSX_ERR_INVALID, "grid too small for the imaging range");
-    }
-    const double img_scale = -1.0 / (sh->dt * sh->dt);
-
-    // ---- buffers
-    Dev traces, res, crec, partial, msum, m1, m2;
-    if ((rc = traces.alloc(ntr * es)) || (rc = res.alloc(ntr * es)) || (rc = crec.alloc(nrec * es)) ||
-        (rc = partial.alloc(4096 * sizeof(double))) || (rc = msum.alloc(sizeof(double))) || (rc = m1.alloc(bytes)) ||
-        (rc = m2.alloc(bytes)))
-        return rc;
-    const int64_t nseg = K == 0 ? 0 : (nt + K - 1) / K;
-    const size_t snap = fmt == HSX_CKPT_FP32 ? bytes : hsx_ckpt_bytes(npoint);
-    std::vector<Dev> q(K == 0 ? nt + 2 : K + 2);   // wavefields q^{m}, m from the segment start - 1
-    for (auto &x : q)
-        if ((rc = x.alloc(bytes))) return rc;
-    Dev ck;
-    if (K > 0 && (rc = ck.alloc((size_t)nseg * 2 * snap))) return rc;
-    Dev fa, fb;   // forward pair when checkpointing
-    if (K > 0 && ((rc = fa.alloc(bytes)) || (rc = fb.alloc(bytes)))) return rc;
-
-    auto save = [&](int64_t seg, const void *cur, const void *prev) -> int {
-        char *dst = ck.c() + (size_t)seg * 2 * snap;
-        if (fmt == HSX_CKPT_FP32) {
-            HSX_HIP(hipMemcpyAsync(dst, cur, bytes, hipMemcpyDeviceToDevice, s));
-            HSX_HIP(hipMemcpyAsync(dst + snap, prev, bytes, hipMemcpyDeviceToDevice, s));
-            return HSX_OK;
-        }
-        int r = hsx_ckpt_encode((const float *)cur, nullptr, npoint, dst, fmt, s);
-        if (!r) r = hsx_ckpt_encode((const float *)cur, (const float *)prev, npoint, dst + snap, fmt, s);
-        return r;
-    };
-    auto restore = [&](int64_t seg, void *cur, void *prev) -> int {
-        const char *src = ck.c() + (size_t)seg * 2 * snap;
-        if (fmt == HSX_CKPT_FP32) {
-            HSX_HIP(hipMemcpyAsync(cur, src, bytes, hipMemcpyDeviceToDevice, s));
-            HSX_HIP(hipMemcpyAsync(prev, src + snap, bytes, hipMemcpyDeviceToDevice, s));
-            return HSX_OK;
-        }
-        int r = hsx_ckpt_decode(src, npoint, fmt, nullptr, (float *)cur, s);
-        if (!r) r = hsx_ckpt_decode(src + snap, npoint, fmt, (const float *)cur, (float *)prev, s);
-        return r;
-    };
-
-    // ---- forward pass
-    if (K == 0) {
-        if ((rc = zero(g, q[0].p, s)) || (rc = zero(g, q[1].p, s))) return rc;
-        for (int64_t n = 0; n < nt; n++)
-            if ((rc = fwd_step(g, coef, sh, q[n + 2].p, q[n].p, q[n + 1].p, n, traces.p, es, s))) return rc;
-    } else {
-        if ((rc = zero(g, fa.p, s)) || (rc = zero(g, fb.p, s))) return rc;
-        void *prev = fa.p, *cur = fb.p;
-        for (int64_t n = 0; n < nt; n++) {
-            if (n % K == 0 && (rc = save(n / K, cur, prev))) return rc;
-            if ((rc = fwd_step(g, coef, sh, prev, prev, cur, n, traces.p, es, s))) return rc;
-            std::swap(prev, cur);
-        }
-    }
-
-    // ---- residual and misfit
-    const int nb = (int)std::max<int64_t>(1, std::min<int64_t>((ntr + 255) / 256, 4096));
-    rc = dispatch_types(g, [&](auto ts, auto) -> int {
-        using TS = decltype(ts);
-        hipLaunchKernelGGL(k_residual<TS>, dim3(nb), dim3(256), 0, s, (const TS *)traces.p, (const TS *)d_obs,
-                           (TS *)res.p, ntr, (double *)partial.p);
-        HSX_LAUNCHED("k_residual");
-        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, s, (const double *)partial.p, nb, (double *)msum.p);
-        HSX_LAUNCHED("k_sum_partials");
-        return HSX_OK;
-    });
-    if (rc) return rc;
-    if (nrec > 0 && (rc = hsx_gather(g, coef, nrec, sh->rec_idx, crec.p, s))) return rc;
-
-    // ---- backward pass: mu^k for k = nt .. 1
-    if ((rc = zero(g, m1.p, s)) || (rc = zero(g, m2.p, s))) return rc;
-    void *mu1 = m1.p, *mu2 = m2.p;   // mu^{k+1}, mu^{k+2}
-    auto back_step = [&](int64_t k, const void *qk, const void *qk1, const void *qk2) -> int {
-        int r = hsx_free_surface(g, mu1, 0x3F, s);
-        if (!r) r = hsx_ac2_step(g, mu2, mu1, coef, s);                    // mu2 <- mu^k
-        if (!r && nrec > 0)
-            r = hsx_inject(g, mu2, nrec, sh->rec_idx, res.c() + (size_t)(k - 1) * nrec * es, crec.p, 1.0,
-                           HSX_INJECT_ADD, s);
-        if (!r) r = hsx_imaging(g, grad, mu2, qk, qk1, qk2, rim, img_scale, -1, 0.0, s);
-        if (!r && sh->nsrc > 0) {
-            r = dispatch_types(g, [&](auto ts, auto) -> int {
-                using TS = decltype(ts);
-                hipLaunchKernelGGL(k_imaging_src<TS>, dim3((unsigned)((sh->nsrc + 63) / 64)), dim3(64), 0, s, grad,
-                                   (const TS *)mu2, sh->nsrc, sh->src_idx,
-                                   (const TS *)((const char *)sh->wavelet + (size_t)(k - 1) * sh->nsrc * es),
-                                   -img_scale);
-                HSX_LAUNCHED("k_imaging_src");
-                return HSX_OK;
-            });
-        }
-        std::swap(mu1, mu2);
-        return r;
-    };
-    if (K == 0) {
-        for (int64_t k = nt; k >= 1; k--)
-            if ((rc = back_step(k, q[k + 1].p, q[k].p, q[k - 1].p))) return rc;
-    } else {
-        for (int64_t seg = nseg - 1; seg >= 0; seg--) {
-            const int64_t n0 = seg * K, n1 = std::min<int64_t>(n0 + K, nt);
-            // q[j] holds q^{n0 - 1 + j}
-            if ((rc = restore(seg, q[1].p, q[0].p))) return rc;
-            for (int64_t n = n0; n < n1; n++)
-                if ((rc = fwd_step(g, coef, sh, q[n - n0 + 2].p, q[n - n0].p, q[n - n0 + 1].p, n, nullptr, es, s)))
-                    return rc;
-            for (int64_t k = n1; k > n0; k--)
-                if ((rc = back_step(k, q[k - n0 + 1].p, q[k - n0].p, q[k - n0 - 1].p))) return rc;
-        }
-    }
-    double m = 0.0;
-    HSX_HIP(hipMemcpyAsync(&m, msum.p, sizeof(double), hipMemcpyDeviceToHost, s));
-    HSX_HIP(hipStreamSynchronize(s));   // buffers are freed on return
-    if (misfit) *misfit = 0.5 * m;
-    return HSX_OK;
+    hsx_adjoint_plan *P = nullptr;
+    if ((rc = hsx_adjoint_plan_create(g, sh->nt, sh->nrec, opt, &P))) return rc;
+    rc = plan_run(P, g, coef, sh, d_obs, grad, misfit, stream_of(stream));
+    hsx_adjoint_plan_destroy(P);   // plan_run has synchronised the stream (or failed)
+    return rc;
 }
 
 } // extern "C"

@@ -21,6 +21,7 @@ struct SrcRec {             // one shot, one step parity
     const int64_t *rec;
     void *traces;
     int64_t nrec;
+    int64_t npoint;         // points of the grid: indices outside [0, npoint) are skipped
 };
 
 // one workgroup per shot: add the source samples of step n, then record
@@ -30,13 +31,18 @@ __global__ void __launch_bounds__(256) k_srcrec_batch(const SrcRec *sr, int64_t 
     const SrcRec &x = sr[blockIdx.x];
     TS *u = reinterpret_cast<TS *>(x.field);
     const TS *w = reinterpret_cast<const TS *>(x.wavelet);
+    const uint64_t np = (uint64_t)x.npoint;
     for (int64_t i = threadIdx.x; i < x.nsrc; i += blockDim.x) {
+        if ((uint64_t)x.src[i] >= np) continue;
         TS *p = u + x.src[i];
         *p = (TS)((double)*p + 1.0 * (double)w[n * x.nsrc + i]);
     }
     __syncthreads();
     TS *tr = reinterpret_cast<TS *>(x.traces);
-    for (int64_t i = threadIdx.x; i < x.nrec; i += blockDim.x) tr[n * x.nrec + i] = u[x.rec[i]];
+    for (int64_t i = threadIdx.x; i < x.nrec; i += blockDim.x) {
+        const int64_t j = x.rec[i];
+        tr[n * x.nrec + i] = (uint64_t)j < np ? u[j] : (TS)0;
+    }
 }
 
 template <typename TS>
@@ -113,16 +119,17 @@ static int batch_setup_shape(hsx_batch *b, const hsx_grid *grids, const void *co
     // 192^3-320^3, order 8: 2.47 s for 1000 steps with 32-plane chunks, 2.75 s
     // with whole columns; 16-40 planes all within 1 %)
     int64_t tiles = 0, chunk = 0;
+    const TileCfg cfg = tile_forced();
     for (int pass = 0; pass < 2; pass++) {
         tiles = 0;
         for (int s = 0; s < ns; s++) {
             void *A = F[s], *B = F[ns + s];
             // parity 0: B is the current wavefield, A the previous one (and the output)
-            int rc = fill_tiled_args<TS, TS, R, VW, LX, BY, MODE_AC2>(&grids[s], A, A, B, coef[s], slots,
+            int rc = fill_tiled_args<TS, TS, R, VW, LX, BY, MODE_AC2>(&grids[s], A, A, B, coef[s], slots, cfg,
                                                                      args[0][s], chunk);
             if (!rc)
-                rc = fill_tiled_args<TS, TS, R, VW, LX, BY, MODE_AC2>(&grids[s], B, B, A, coef[s], slots, args[1][s],
-                                                                     chunk);
+                rc = fill_tiled_args<TS, TS, R, VW, LX, BY, MODE_AC2>(&grids[s], B, B, A, coef[s], slots, cfg,
+                                                                     args[1][s], chunk);
             if (rc) return rc;
             off[s + 1] = off[s] + args[0][s].ntx * args[0][s].nty * args[0][s].nchunk;
             tiles += (int64_t)args[0][s].ntx * args[0][s].nty;
@@ -291,6 +298,7 @@ int hsx_batch_create(int nshots, const hsx_grid *grids, const void *const *coef,
             x.rec = shots[s].rec_idx;
             x.traces = traces[s];
             x.nrec = shots[s].nrec;
+            x.npoint = n;
         }
     }
     if ((rc = upload(b, F.data(), F.size() * sizeof(void *), &b->fields)) ||

@@ -652,6 +652,16 @@ __global__ void __launch_bounds__(256) k_reduce_final(const double *part, int nb
 
 static std::mutex g_red_mu;   // the partial-sum scratch is shared by all callers
 
+// partial results per reduction: one block per ROWS_PER_BLOCK rows, at most
+// RED_BLOCKS blocks (the rows of larger ranges are strided over them)
+constexpr int RED_BLOCKS = 1024;
+static int red_blocks(const Box &b)
+{
+    const int64_t rows = b.n2() * b.n3();
+    const int64_t nb = (rows + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK;
+    return (int)(nb < RED_BLOCKS ? nb : RED_BLOCKS);
+}
+
 template <int OP>
 static int run_reduce(const hsx_grid *g, const void *u, const void *v, const int64_t r[6], float maxerr,
                       double out[2], void *stream)
@@ -665,9 +675,7 @@ static int run_reduce(const hsx_grid *g, const void *u, const void *v, const int
     if (rc) return rc;
     std::lock_guard<std::mutex> lk(g_red_mu);
     const Box b = box_of(r);
-    const int64_t rows = b.n2() * b.n3();
-    int nb = (int)((rows + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK);
-    if (nb > 1024) nb = 1024;
+    const int nb = red_blocks(b);
     double *part = nullptr, *res = nullptr;
     if ((rc = scratch_get(SCRATCH_PARTIALS, sizeof(double) * 2 * nb, (void **)&part))) return rc;
     if ((rc = scratch_get(SCRATCH_RESULT, sizeof(double) * 2, (void **)&res))) return rc;

@@ -11,12 +11,14 @@
 namespace hsx {
 
 // ---- sources / receivers ------------------------------------------------------------
-// u[idx[i]] (+)= scale * val[i] * (w ? w[i] : 1); indices are distinct within one call
+// u[idx[i]] (+)= scale * val[i] * (w ? w[i] : 1); indices are distinct within one call;
+// an index outside [0, npoint) is skipped (the host checks them where it builds them)
 template <typename TS>
-__global__ void __launch_bounds__(256) k_inject(TS *u, int64_t n, const int64_t *idx, const TS *val, const TS *w,
-                                                double scale, int mode)
+__global__ void __launch_bounds__(256) k_inject(TS *u, int64_t npoint, int64_t n, const int64_t *idx, const TS *val,
+                                                const TS *w, double scale, int mode)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if ((uint64_t)idx[i] >= (uint64_t)npoint) continue;
         double v = scale * (double)val[i];
         if (w) v *= (double)w[i];
         TS *p = u + idx[i];
@@ -24,11 +26,14 @@ __global__ void __launch_bounds__(256) k_inject(TS *u, int64_t n, const int64_t 
     }
 }
 
+// out[i] = u[idx[i]]; 0 for an index outside [0, npoint)
 template <typename TS>
-__global__ void __launch_bounds__(256) k_gather(const TS *u, int64_t n, const int64_t *idx, TS *out)
+__global__ void __launch_bounds__(256) k_gather(const TS *u, int64_t npoint, int64_t n, const int64_t *idx, TS *out)
 {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        out[i] = u[idx[i]];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = idx[i];
+        out[i] = (uint64_t)j < (uint64_t)npoint ? u[j] : (TS)0;
+    }
 }
 
 // ---- imaging condition -----------------------------------------------------------
@@ -135,7 +140,10 @@ using namespace hsx;
 
 extern "C" {
 
-size_t hsx_ckpt_bytes(int64_t n) { return CKPT_HEADER + (size_t)n * 2; }
+// header + codes, rounded up to whole 256-byte units: snapshots stored back to
+// back keep every header 256-byte aligned (its max |value| is updated with a
+// 32-bit atomic)
+size_t hsx_ckpt_bytes(int64_t n) { return CKPT_HEADER + ((size_t)n * 2 + CKPT_HEADER - 1) / CKPT_HEADER * CKPT_HEADER; }
 
 int hsx_inject(const hsx_grid *g, void *u, int64_t n, const int64_t *idx, const void *val, const void *w,
                double scale, int mode, void *stream)
@@ -150,7 +158,8 @@ int hsx_inject(const hsx_grid *g, void *u, int64_t n, const int64_t *idx, const 
     hipStream_t s = stream_of(stream);
     return dispatch_types(g, [&](auto ts, auto) -> int {
         using TS = decltype(ts);
-        hipLaunchKernelGGL(k_inject<TS>, dim3(blocks_for(n)), dim3(256), 0, s, (TS *)u, n, idx, (const TS *)val,
+        hipLaunchKernelGGL(k_inject<TS>, dim3(blocks_for(n)), dim3(256), 0, s, (TS *)u, g->n1 * g->n2 * g->n3, n, idx,
+                           (const TS *)val,
                            (const TS *)w, scale, mode);
         HSX_LAUNCHED("k_inject");
         return HSX_OK;
@@ -168,7 +177,8 @@ int hsx_gather(const hsx_grid *g, const void *u, int64_t n, const int64_t *idx, 
     hipStream_t s = stream_of(stream);
     return dispatch_types(g, [&](auto ts, auto) -> int {
         using TS = decltype(ts);
-        hipLaunchKernelGGL(k_gather<TS>, dim3(blocks_for(n)), dim3(256), 0, s, (const TS *)u, n, idx, (TS *)out);
+        hipLaunchKernelGGL(k_gather<TS>, dim3(blocks_for(n)), dim3(256), 0, s, (const TS *)u, g->n1 * g->n2 * g->n3, n,
+                           idx, (TS *)out);
         HSX_LAUNCHED("k_gather");
         return HSX_OK;
     });
@@ -203,6 +213,7 @@ int hsx_ckpt_encode(const float *u, const float *u_prev, int64_t n, void *packed
     if ((rc = check_ptr(u, "u")) || (rc = check_ptr(packed, "packed"))) return rc;
     if (format != HSX_CKPT_FP16 && format != HSX_CKPT_BF16) return fail(HSX_ERR_INVALID, "format %d", format);
     if (n <= 0) return fail(HSX_ERR_INVALID, "n = %lld", (long long)n);
+    if ((uintptr_t)packed % 16) return fail(HSX_ERR_INVALID, "packed snapshot not 16-byte aligned");
     if ((rc = ensure_init())) return rc;
     hipStream_t s = stream_of(stream);
     unsigned *hdr = reinterpret_cast<unsigned *>(packed);
@@ -221,6 +232,7 @@ int hsx_ckpt_decode(const void *packed, int64_t n, int format, const float *base
     if ((rc = check_ptr(packed, "packed")) || (rc = check_ptr(out, "out"))) return rc;
     if (format != HSX_CKPT_FP16 && format != HSX_CKPT_BF16) return fail(HSX_ERR_INVALID, "format %d", format);
     if (n <= 0) return fail(HSX_ERR_INVALID, "n = %lld", (long long)n);
+    if ((uintptr_t)packed % 16) return fail(HSX_ERR_INVALID, "packed snapshot not 16-byte aligned");
     if ((rc = ensure_init())) return rc;
     hipStream_t s = stream_of(stream);
     hipLaunchKernelGGL(k_ckpt_decode, dim3(blocks_for(n)), dim3(256), 0, s, (const char *)packed, n, format, base,

@@ -653,7 +653,10 @@ __global__ void __launch_bounds__(LX *BY) k_tiled_batch(const TiledArgs<TC> *__r
 struct TileCfg {
     int lx, by, chunk, flags;
 };
-extern TileCfg g_tile;                // defined in hsx_tiled_ac2.hip
+// the shape forced by hsx_set_tile (a copy, read under its lock); every launch
+// takes its shape as a value, so that threads stepping different grids or
+// tuning never see each other's choice
+TileCfg tile_forced(void);            // defined in hsx_tiled_ac2.hip
 
 // free surface folded into the next MODE_AC2 launch of this thread (hsx_ac2_step_fs)
 struct FoldReq {
@@ -668,9 +671,9 @@ constexpr int TILE_LDS_MAX = 163840;  // gfx950: 160 KiB of LDS per workgroup
 // 8 ran 0.125 ms with 32 x 16 tiles against 0.150 ms with 64 x 16 (512 lanes'
 // width for 320 points); 608^3 and 808^3 stay faster with 64 lanes (0.75 /
 // 1.81 ms against 0.93 / 2.22 ms) although a fifth of their lanes idle
-inline int pick_lx(const hsx_grid *g, int vw)
+inline int pick_lx(const hsx_grid *g, int vw, const TileCfg &cfg)
 {
-    if (g_tile.lx) return g_tile.lx;
+    if (cfg.lx) return cfg.lx;
     const int64_t N1 = g->inner[1] - g->inner[0] + 1;
     if (N1 <= 16 * vw) return 16;
     if (N1 <= 32 * vw) return 32;
@@ -751,7 +754,7 @@ inline bool tiled_spills(KERN kern)
 // (chunk > 0: n3 planes per workgroup, at least 4R, unless hsx_set_tile forces one)
 template <typename TS, typename TC, int R, int VW, int LX, int BY, int MODE, bool RING = false>
 inline int fill_tiled_args(const hsx_grid *g, void *out, const void *prn_in, const void *u, const void *coef,
-                           int64_t slots, TiledArgs<TC> &a, int64_t chunk = 0)
+                           int64_t slots, const TileCfg &cfg, TiledArgs<TC> &a, int64_t chunk = 0)
 {
     a.out = out;
     a.prn_in = prn_in ? prn_in : out;
@@ -774,13 +777,13 @@ inline int fill_tiled_args(const hsx_grid *g, void *out, const void *prn_in, con
     a.nty = (int)((a.e2 - a.s2 + 1 + BY - 1) / BY);
     const int64_t N3 = a.e3 - a.s3 + 1;
     const int64_t tiles = (int64_t)a.ntx * a.nty;
-    const int64_t len = g_tile.chunk > 0 ? std::min<int64_t>(g_tile.chunk, N3)
+    const int64_t len = cfg.chunk > 0 ? std::min<int64_t>(cfg.chunk, N3)
                         : chunk > 0           ? std::min<int64_t>(std::max<int64_t>(chunk, 4 * R), N3)
                         : (BY == 16 || RING) ? chunk_one_round(N3, R, tiles, slots)
                                           : chunk_short(N3, R, tiles);
     a.chunk = (int)len;
     a.nchunk = (int)((N3 + len - 1) / len);
-    const int streams = g_tile.flags & (HSX_TILE_CACHED | HSX_TILE_NT);
+    const int streams = cfg.flags & (HSX_TILE_CACHED | HSX_TILE_NT);
     a.nt = streams == HSX_TILE_NT || (streams == 0 && (BY == 16 || RING));
     a.fold = MODE == MODE_AC2 ? t_fold.flags : 0;
     for (int k = 0; k < 6; k++) a.fb[k] = a.fold ? (int)t_fold.fb[k] : -1;
@@ -790,14 +793,16 @@ inline int fill_tiled_args(const hsx_grid *g, void *out, const void *prn_in, con
 }
 
 template <typename TS, typename TC, int R, int VW, int LX, int BY, int MODE, bool RING, bool FOLD = false>
-inline int launch_tiled(const hsx_grid *g, void *out, const void *prn_in, const void *u, const void *coef, hipStream_t s)
+inline int launch_tiled(const hsx_grid *g, void *out, const void *prn_in, const void *u, const void *coef, hipStream_t s,
+                        const TileCfg &cfg)
 {
     using K = Tiled<TS, TC, R, VW, LX, BY, MODE, RING>;
     static_assert(K::LDS <= TILE_LDS_MAX, "LDS tile above the workgroup limit");
     auto kern = k_tiled<TS, TC, R, VW, LX, BY, MODE, RING, FOLD>;
     static const int per_cu = tiled_per_cu(kern, LX * BY, K::LDS);
     TiledArgs<TC> a;
-    int rc = fill_tiled_args<TS, TC, R, VW, LX, BY, MODE, RING>(g, out, prn_in, u, coef, (int64_t)per_cu * device_cus(), a);
+    int rc = fill_tiled_args<TS, TC, R, VW, LX, BY, MODE, RING>(g, out, prn_in, u, coef, (int64_t)per_cu * device_cus(),
+                                                                cfg, a);
     if (rc) return rc;
     const int64_t nblk = (int64_t)a.ntx * a.nty * a.nchunk;
     const unsigned grid = (unsigned)(((nblk + 7) / 8) * 8);
@@ -820,15 +825,15 @@ struct TallFits {
 };
 
 template <typename TS, typename TC, int R, int VW, int LX, int MODE, bool FOLD = false>
-inline int tall_variant()
+inline int tall_variant(const TileCfg &cfg)
 {
     constexpr bool ring_ok = TallFits<TS, TC, R, VW, LX, MODE>::ring;
     constexpr bool tall_ok = TallFits<TS, TC, R, VW, LX, MODE>::tall;
-    const bool forced = g_tile.by == 16;
-    if (g_tile.by == 8) return 0;
+    const bool forced = cfg.by == 16;
+    if (cfg.by == 8) return 0;
     if constexpr (ring_ok) {
         static const bool ring_spills = tiled_spills(k_tiled<TS, TC, R, VW, LX, 16, MODE, true, FOLD>);
-        if (!(g_tile.flags & HSX_TILE_NO_RING) && (forced || !ring_spills)) return 1;
+        if (!(cfg.flags & HSX_TILE_NO_RING) && (forced || !ring_spills)) return 1;
     }
     if constexpr (tall_ok) {
         static const bool tall_spills = tiled_spills(k_tiled<TS, TC, R, VW, LX, 16, MODE, false, FOLD>);
@@ -839,14 +844,15 @@ inline int tall_variant()
 
 // BY = 16 (with the halo ring where it fits) for tiles of at least 32 lanes, else 8
 template <typename TS, typename TC, int R, int VW, int LX, int MODE, bool FOLD = false>
-inline int launch_tiled_by(const hsx_grid *g, void *out, const void *prn_in, const void *u, const void *coef, hipStream_t s)
+inline int launch_tiled_by(const hsx_grid *g, void *out, const void *prn_in, const void *u, const void *coef, hipStream_t s,
+                           const TileCfg &cfg)
 {
-    const int tall = tall_variant<TS, TC, R, VW, LX, MODE, FOLD>();
+    const int tall = tall_variant<TS, TC, R, VW, LX, MODE, FOLD>(cfg);
     if constexpr (TallFits<TS, TC, R, VW, LX, MODE>::ring) {
-        if (tall == 1) return launch_tiled<TS, TC, R, VW, LX, 16, MODE, true, FOLD>(g, out, prn_in, u, coef, s);
+        if (tall == 1) return launch_tiled<TS, TC, R, VW, LX, 16, MODE, true, FOLD>(g, out, prn_in, u, coef, s, cfg);
     }
     if constexpr (TallFits<TS, TC, R, VW, LX, MODE>::tall) {
-        if (tall == 2) return launch_tiled<TS, TC, R, VW, LX, 16, MODE, false, FOLD>(g, out, prn_in, u, coef, s);
+        if (tall == 2) return launch_tiled<TS, TC, R, VW, LX, 16, MODE, false, FOLD>(g, out, prn_in, u, coef, s, cfg);
     }
     // 8 rows: the ring by default for FP64 storage only (1000^3 fused step,
     // 64 x 8: FP64 6.88 / 7.26 / 7.48 ms at orders 4 / 8 / 12 with the ring and
@@ -857,11 +863,11 @@ inline int launch_tiled_by(const hsx_grid *g, void *out, const void *prn_in, con
     constexpr bool ring8_ok = Tiled<TS, TC, R, VW, LX, 8, MODE, true>::RING &&
                               Tiled<TS, TC, R, VW, LX, 8, MODE, true>::LDS <= TILE_LDS_MAX;
     if constexpr (ring8_ok && !FOLD) {
-        const bool ring8 = g_tile.by == 8 || (g_tile.by == 0 && sizeof(TS) == 8);
-        if (ring8 && !(g_tile.flags & HSX_TILE_NO_RING))
-            return launch_tiled<TS, TC, R, VW, LX, 8, MODE, true>(g, out, prn_in, u, coef, s);
+        const bool ring8 = cfg.by == 8 || (cfg.by == 0 && sizeof(TS) == 8);
+        if (ring8 && !(cfg.flags & HSX_TILE_NO_RING))
+            return launch_tiled<TS, TC, R, VW, LX, 8, MODE, true>(g, out, prn_in, u, coef, s, cfg);
     }
-    return launch_tiled<TS, TC, R, VW, LX, 8, MODE, false, FOLD>(g, out, prn_in, u, coef, s);
+    return launch_tiled<TS, TC, R, VW, LX, 8, MODE, false, FOLD>(g, out, prn_in, u, coef, s, cfg);
 }
 
 // alignment and size conditions of the tiled kernels
@@ -878,7 +884,7 @@ bool tuned_shape(const hsx_grid *g, int mode, TileCfg *out);
 
 template <int MODE, bool FOLD = false>
 inline int run_tiled_shape(const hsx_grid *g, void *out, const void *prn_in, const void *u, const void *coef,
-                           hipStream_t s)
+                           hipStream_t s, const TileCfg &cfg)
 {
     const int R = g->fd_order / 2;
     return dispatch_types(g, [&](auto ts, auto tc) -> int {
@@ -889,10 +895,11 @@ inline int run_tiled_shape(const hsx_grid *g, void *out, const void *prn_in, con
         if (rc) return rc;
         if (((uintptr_t)out | (uintptr_t)prn_in | (uintptr_t)u | (uintptr_t)coef) % 16)
             return fail(HSX_ERR_UNSUPPORTED, "buffers not 16-byte aligned");
-        const int lx = pick_lx(g, VW);
+        const int lx = pick_lx(g, VW, cfg);
         return dispatch_R(R, [&](auto rc_) -> int {
             constexpr int RR = decltype(rc_)::value;
-            if (lx == 16) return launch_tiled<TS, TC, RR, VW, 16, 8, MODE, false, FOLD>(g, out, prn_in, u, coef, s);
+            if (lx == 16)
+                return launch_tiled<TS, TC, RR, VW, 16, 8, MODE, false, FOLD>(g, out, prn_in, u, coef, s, cfg);
             // 64 lanes without a 16-row instance that keeps its values in
             // registers: FP64 values up to order 12 take 64 x 8 tiles, the rest
             // (order 16 FP32, orders 14-16 in FP64) 32 lanes x 16 rows.
@@ -902,11 +909,12 @@ inline int run_tiled_shape(const hsx_grid *g, void *out, const void *prn_in, con
             // 4.50 ms for 64 x 16 with 8-byte lanes, which fits in registers
             // but issues twice the loads and LDS reads per point)
             constexpr bool fp64_vals = !(sizeof(TS) == 4 && sizeof(TC) == 4);
-            const bool narrow = lx == 64 && g_tile.lx == 0 && !(fp64_vals && RR <= 6) &&
-                                tall_variant<TS, TC, RR, VW, 64, MODE, FOLD>() == 0 &&
-                                tall_variant<TS, TC, RR, VW, 32, MODE, FOLD>() != 0;
-            if (lx == 32 || narrow) return launch_tiled_by<TS, TC, RR, VW, 32, MODE, FOLD>(g, out, prn_in, u, coef, s);
-            return launch_tiled_by<TS, TC, RR, VW, 64, MODE, FOLD>(g, out, prn_in, u, coef, s);
+            const bool narrow = lx == 64 && cfg.lx == 0 && !(fp64_vals && RR <= 6) &&
+                                tall_variant<TS, TC, RR, VW, 64, MODE, FOLD>(cfg) == 0 &&
+                                tall_variant<TS, TC, RR, VW, 32, MODE, FOLD>(cfg) != 0;
+            if (lx == 32 || narrow)
+                return launch_tiled_by<TS, TC, RR, VW, 32, MODE, FOLD>(g, out, prn_in, u, coef, s, cfg);
+            return launch_tiled_by<TS, TC, RR, VW, 64, MODE, FOLD>(g, out, prn_in, u, coef, s, cfg);
         });
     });
 }
@@ -915,14 +923,11 @@ inline int run_tiled_shape(const hsx_grid *g, void *out, const void *prn_in, con
 template <int MODE, bool FOLD = false>
 inline int run_tiled(const hsx_grid *g, void *out, const void *prn_in, const void *u, const void *coef, hipStream_t s)
 {
-    const TileCfg forced = g_tile;
+    const TileCfg forced = tile_forced();
     TileCfg tuned;
     const bool use_tuned = forced.lx == 0 && forced.by == 0 && forced.chunk == 0 && forced.flags == 0 &&
                            tuned_shape(g, MODE, &tuned);
-    if (use_tuned) g_tile = tuned;
-    const int rc = run_tiled_shape<MODE, FOLD>(g, out, prn_in, u, coef, s);
-    if (use_tuned) g_tile = forced;
-    return rc;
+    return run_tiled_shape<MODE, FOLD>(g, out, prn_in, u, coef, s, use_tuned ? tuned : forced);
 }
 
 } // namespace hsx

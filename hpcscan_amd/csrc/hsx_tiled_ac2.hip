@@ -9,7 +9,14 @@
 
 namespace hsx {
 
-TileCfg g_tile = {0, 0, 0, 0};
+static std::mutex g_tile_mu;
+static TileCfg g_tile = {0, 0, 0, 0};   // hsx_set_tile
+
+TileCfg tile_forced(void)
+{
+    std::lock_guard<std::mutex> lk(g_tile_mu);
+    return g_tile;
+}
 
 int set_tile(int lx, int by, int chunk, int flags)
 {
@@ -21,6 +28,7 @@ int set_tile(int lx, int by, int chunk, int flags)
         return fail(HSX_ERR_INVALID, "tile flags 0x%x", flags);
     if ((flags & HSX_TILE_CACHED) && (flags & HSX_TILE_NT))
         return fail(HSX_ERR_INVALID, "tile flags: cached and nontemporal streams together");
+    std::lock_guard<std::mutex> lk(g_tile_mu);
     g_tile = TileCfg{lx, by, chunk, flags};
     return HSX_OK;
 }
@@ -104,24 +112,25 @@ extern "C" int hsx_autotune(const hsx_grid *g, int reps, int shape[4], float *be
     }
     const int64_t N3 = g->inner[5] - g->inner[4] + 1;
     const int R = g->fd_order / 2;
-    // candidates: lanes x rows, stream mode, halo ring, n3 chunk
+    // candidates: lanes x rows, stream mode, halo ring, n3 chunk; the first is
+    // the library's own choice, so the shape kept is never slower than it in
+    // this sweep
     std::vector<TileCfg> cand = {
+        {0, 0, 0, 0},
         {64, 16, 0, 0}, {64, 16, 0, HSX_TILE_NO_RING}, {64, 16, 0, HSX_TILE_CACHED},
         {32, 16, 0, 0}, {32, 16, 0, HSX_TILE_NO_RING},
         {64, 8, 0, 0}, {64, 8, 0, HSX_TILE_NO_RING}, {64, 8, 0, HSX_TILE_NO_RING | HSX_TILE_NT},
         {64, 8, (int)std::min<int64_t>(N3, 8 * R + 64), HSX_TILE_NO_RING},
         {32, 8, 0, 0}, {32, 8, 0, HSX_TILE_NO_RING}, {16, 8, 0, 0},
     };
-    const TileCfg saved = g_tile;
     TileCfg best = {0, 0, 0, 0};
     float best_t = 1e30f;
     for (const TileCfg &c : cand) {
-        g_tile = c;
-        rc = run_tiled_shape<MODE_AC2>(g, buf[0], buf[0], buf[1], buf[2], s);   // warm-up (and fit check)
+        rc = run_tiled_shape<MODE_AC2>(g, buf[0], buf[0], buf[1], buf[2], s, c);   // warm-up (and fit check)
         if (rc == HSX_ERR_UNSUPPORTED) continue;
         if (rc) break;
         (void)hipEventRecord(e0, s);
-        for (int k = 0; k < reps && !rc; k++) rc = run_tiled_shape<MODE_AC2>(g, buf[0], buf[0], buf[1], buf[2], s);
+        for (int k = 0; k < reps && !rc; k++) rc = run_tiled_shape<MODE_AC2>(g, buf[0], buf[0], buf[1], buf[2], s, c);
         (void)hipEventRecord(e1, s);
         if (rc) break;
         (void)hipEventSynchronize(e1);
@@ -133,7 +142,6 @@ extern "C" int hsx_autotune(const hsx_grid *g, int reps, int shape[4], float *be
             best = c;
         }
     }
-    g_tile = saved;
     (void)hipStreamSynchronize(s);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

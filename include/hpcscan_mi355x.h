@@ -217,7 +217,9 @@ int hsx_imaging(const hsx_grid *g, double *grad, const void *mu, const void *u_n
 /* ---- low-precision checkpoints of FP32 wavefields ----------------------------
  * A snapshot of n values takes hsx_ckpt_bytes(n) bytes: a 256-byte header
  * (scale, max |value|) and n 16-bit codes, code = round(v / scale) with
- * scale = max|v| / 16384 (FP16) or max|v| (BF16).  With u_prev non-NULL the
+ * scale = max|v| / 16384 (FP16) or max|v| (BF16), padded to a multiple of 256
+ * bytes so that snapshots stored back to back stay aligned.  packed must be
+ * 16-byte aligned (HSX_ERR_INVALID otherwise).  With u_prev non-NULL the
  * snapshot holds u - u_prev; decoding with base non-NULL gives base - value. */
 #define HSX_CKPT_FP32 0   /* checkpoints kept in the storage type (bit-exact recomputation) */
 #define HSX_CKPT_FP16 1
@@ -243,18 +245,37 @@ typedef struct {
 
 int hsx_forward(const hsx_grid *g, const void *coef, const hsx_shot *shot, void *traces, void *stream);
 
+/* d_obs is the residual itself (the adjoint source, e.g. the gradient of any
+ * loss with respect to the traces): res = d_obs instead of traces - d_obs, so
+ * that it reaches the adjoint without being rounded against the traces */
+#define HSX_ADJ_RESIDUAL 1
+
 typedef struct {
     int ckpt_every;            /* steps per checkpoint segment; 0 = keep every wavefield */
     int ckpt_format;           /* HSX_CKPT_FP32 / FP16 / BF16 (16-bit: FP32 grids only) */
+    int flags;                 /* 0 or HSX_ADJ_RESIDUAL */
 } hsx_adjoint_opts;
 
-/* grad (DEVICE, double, npoint values) += dJ/dm for J = 1/2 sum (traces - d_obs)^2,
- * m = dt^2 / coef the squared slowness, over the inner points not on a
- * boundary plane; *misfit = J (may be NULL).  Forward pass with checkpoints,
- * then the adjoint backwards, recomputing one segment at a time.  Allocates its
- * work buffers and returns after the stream has finished with them. */
+/* grad (DEVICE, double, npoint values) += dJ/dm for J = 1/2 sum res^2,
+ * res = traces - d_obs (or d_obs with HSX_ADJ_RESIDUAL), m = dt^2 / coef the
+ * squared slowness, over the inner points not on a boundary plane (source
+ * terms outside that box included: they add nothing); *misfit = J (may be
+ * NULL).  Forward pass with checkpoints, then the adjoint backwards,
+ * recomputing one segment at a time.  Allocates its work buffers (a plan, see
+ * below) and returns after the stream has finished with them. */
 int hsx_gradient(const hsx_grid *g, const void *coef, const hsx_shot *shot, const void *d_obs,
                  const hsx_adjoint_opts *opts, double *grad, double *misfit, void *stream);
+
+/* The work buffers of hsx_gradient for one grid layout, nt, receiver count
+ * and options, allocated once: hsx_gradient_plan then runs gradients of shots
+ * of that geometry (any sources, receiver positions, coef, data) without
+ * allocating.  A plan serves one call at a time. */
+typedef struct hsx_adjoint_plan hsx_adjoint_plan;
+int hsx_adjoint_plan_create(const hsx_grid *g, int64_t nt, int64_t nrec, const hsx_adjoint_opts *opts,
+                            hsx_adjoint_plan **out);
+int hsx_gradient_plan(hsx_adjoint_plan *plan, const hsx_grid *g, const void *coef, const hsx_shot *shot,
+                      const void *d_obs, double *grad, double *misfit, void *stream);
+int hsx_adjoint_plan_destroy(hsx_adjoint_plan *plan);
 
 /* ---- batched shots (no counterpart in the reference: one grid per process) ----
  * nshots forward modelings (hsx_forward of each shot, same results) advanced

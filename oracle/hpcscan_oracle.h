@@ -120,7 +120,7 @@ int    orc_forward_##SFX(const orc_grid *g, const TS *coef, const TC inv2[3], in
 int    orc_gradient_##SFX(const orc_grid *g, const TS *coef, const TC inv2[3], double dt,   \
                           int64_t nt, int64_t nsrc, const int64_t *src_idx,                \
                           const TS *wavelet, int64_t nrec, const int64_t *rec_idx,         \
-                          const TS *d_obs, double *grad, double *misfit);
+                          const TS *d_obs, int residual_given, double *grad, double *misfit);
 
 ORC_DECLARE(f32, float, float)
 ORC_DECLARE(f64, double, double)

@@ -110,7 +110,7 @@ def lib():
             i64 = ctypes.c_int64
             getattr(L, f"orc_forward_{sfx}").argtypes = [P(OrcGrid), vp, vp, i64, i64, vp, vp, i64, vp, vp]
             getattr(L, f"orc_gradient_{sfx}").argtypes = [P(OrcGrid), vp, vp, ctypes.c_double, i64, i64, vp,
-                                                          vp, i64, vp, vp, vp, P(ctypes.c_double)]
+                                                          vp, i64, vp, vp, ctypes.c_int, vp, P(ctypes.c_double)]
             for red in ("sum_abs", "min", "max"):
                 f = getattr(L, f"orc_{red}_{sfx}")
                 f.argtypes = [P(OrcGrid), vp, P(ctypes.c_int64)]
@@ -414,8 +414,9 @@ def forward(g, coef, iv, nt, src_idx, wavelet, rec_idx, compute64=False):
     return traces
 
 
-def gradient(g, coef, iv, dt, nt, src_idx, wavelet, rec_idx, d_obs, compute64=False):
-    """dJ/dm (double, npoint) and J for J = 1/2 |traces - d_obs|^2, full storage."""
+def gradient(g, coef, iv, dt, nt, src_idx, wavelet, rec_idx, d_obs, compute64=False, residual=False):
+    """dJ/dm (double, npoint) and J for J = 1/2 |traces - d_obs|^2, full storage;
+    residual=True: d_obs is the residual itself (the adjoint source)."""
     src_idx = np.ascontiguousarray(src_idx, np.int64)
     rec_idx = np.ascontiguousarray(rec_idx, np.int64)
     wavelet = np.ascontiguousarray(wavelet, coef.dtype)
@@ -424,7 +425,7 @@ def gradient(g, coef, iv, dt, nt, src_idx, wavelet, rec_idx, d_obs, compute64=Fa
     mis = ctypes.c_double(0.0)
     rc = getattr(lib(), f"orc_gradient_{_sfx(coef.dtype, compute64)}")(
         ctypes.byref(g), _p(coef), _p(iv), float(dt), nt, len(src_idx), _p(src_idx), _p(wavelet),
-        len(rec_idx), _p(rec_idx), _p(d_obs), _p(grad), ctypes.byref(mis))
+        len(rec_idx), _p(rec_idx), _p(d_obs), 1 if residual else 0, _p(grad), ctypes.byref(mis))
     if rc:
         raise MemoryError("oracle gradient: allocation failed")
     return grad, mis.value

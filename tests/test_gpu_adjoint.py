@@ -224,3 +224,91 @@ def test_batched_shots_several_rounds(oracle, capi, fp64):
         ref = oracle.forward(sh["o"], sh["coef"], sh["iv"], sh["nt"], sh["src"], sh["wavelet"], sh["rec"])
         _same(tr.download(sh["dtype"]).reshape(ref.shape), ref)
     b.close()
+
+
+@pytest.mark.parametrize("fmt", ["fp16", "bf16"])
+def test_16bit_checkpoints_odd_point_count(oracle, capi, fmt):
+    """an odd number of grid points (41 x 37 x 33 inner, order 8: 49 x 45 x 41
+    points): every snapshot after the first starts at an odd multiple of
+    2 bytes unless the snapshot size is padded; the gradient is the one of
+    aligned snapshots (against the FP64 oracle, and equal to storage-type
+    checkpoints within the 16-bit error)"""
+    n = (41, 37, 33)
+    sh64 = make_shot(oracle, n, 8, nt=60, fp64=True, seed=23)
+    sh = make_shot(oracle, n, 8, nt=60, fp64=False, seed=23)
+    assert sh["p"].npoint % 2 == 1
+    assert capi.ckpt_bytes(sh["p"].npoint) % 256 == 0
+    g64, _ = oracle.gradient(sh64["o"], sh64["coef"], sh64["iv"], sh64["dt"], sh64["nt"], sh64["src"],
+                             sh64["wavelet"], sh64["rec"], sh64["d_obs"])
+    g = capi.hsx_grid(sh["p"])
+    d, s = _shot_dev(capi, sh)
+    dobs = _dev(capi, sh["d_obs"])
+    f = {"fp16": capi.HSX_CKPT_FP16, "bf16": capi.HSX_CKPT_BF16}[fmt]
+    grad = _dev(capi, np.zeros(sh["p"].npoint))
+    capi.gradient(g, d["coef"], s, dobs, grad, 7, f)
+    x = grad.download(np.float64)
+    err = float(np.linalg.norm(x - g64) / np.linalg.norm(g64))
+    print(f"{fmt} checkpoints every 7, odd npoint {sh['p'].npoint}: rel L2 vs FP64 oracle {err:.3e}", flush=True)
+    assert np.all(x[~sh["grad_mask"]] == 0)
+    assert err < {"fp16": 1e-3, "bf16": 1e-2}[fmt]
+
+
+def test_plan_and_given_residual(oracle, capi):
+    """one plan serves two shots of one geometry; a residual passed in
+    (HSX_ADJ_RESIDUAL) gives the gradient of the formed residual bit for bit"""
+    sh = make_shot(oracle, (24, 20, 18), 8, nt=45, seed=27)
+    g = capi.hsx_grid(sh["p"])
+    d, s = _shot_dev(capi, sh)
+    plan = capi.AdjointPlan(g, sh["nt"], len(sh["rec"]), 9, capi.HSX_CKPT_FP32)
+    rplan = capi.AdjointPlan(g, sh["nt"], len(sh["rec"]), 9, capi.HSX_CKPT_FP32, residual=True)
+    trace = oracle.forward(sh["o"], sh["coef"], sh["iv"], sh["nt"], sh["src"], sh["wavelet"], sh["rec"])
+    for k, dobs_h in enumerate((sh["d_obs"], (sh["d_obs"] * 0.5).astype(np.float32))):
+        gref, Jref = oracle.gradient(sh["o"], sh["coef"], sh["iv"], sh["dt"], sh["nt"], sh["src"], sh["wavelet"],
+                                     sh["rec"], dobs_h)
+        grad = _dev(capi, np.zeros(sh["p"].npoint))
+        J = plan.gradient(g, d["coef"], s, _dev(capi, dobs_h), grad)
+        assert J == pytest.approx(Jref, rel=1e-12)
+        _same(grad.download(np.float64), gref)
+        res = (trace - dobs_h).astype(np.float32)
+        grad_r = _dev(capi, np.zeros(sh["p"].npoint))
+        J_r = rplan.gradient(g, d["coef"], s, _dev(capi, res), grad_r)
+        assert J_r == pytest.approx(Jref, rel=1e-12), k
+        _same(grad_r.download(np.float64), gref)
+    # a plan refuses another geometry
+    sh2 = make_shot(oracle, (24, 20, 20), 8, nt=45, seed=27, with_obs=False)
+    d2, s2 = _shot_dev(capi, sh2)
+    with pytest.raises(capi.HsxError):
+        plan.gradient(capi.hsx_grid(sh2["p"]), d2["coef"], s2, _dev(capi, sh["d_obs"]),
+                      _dev(capi, np.zeros(sh2["p"].npoint)))
+    plan.close()
+    rplan.close()
+
+
+def test_source_on_a_boundary_plane_bit_exact(oracle, capi):
+    """sources on and off the n1 boundary plane: the term of a source the free
+    surface zeroes is left out of the imaging condition, as in the oracle"""
+    sh = make_shot(oracle, (24, 20, 18), 8, nt=40, seed=29)
+    p = sh["p"]
+    r = p.inner()
+    mid2, mid3 = (r[2] + r[3]) // 2, (r[4] + r[5]) // 2
+    sh["src"] = np.array([r[0] + mid2 * p.n1 + mid3 * p.n1 * p.n2, (r[0] + 3) + (mid2 + 1) * p.n1 + mid3 * p.n1 * p.n2],
+                         np.int64)
+    sh["wavelet"] = np.repeat(sh["wavelet"], 2, axis=1).copy()
+    d_obs = oracle.forward(sh["o"], sh["coef_true"], sh["iv"], sh["nt"], sh["src"], sh["wavelet"], sh["rec"])
+    g = capi.hsx_grid(p)
+    d, s = _shot_dev(capi, sh)
+    grad = _dev(capi, np.zeros(p.npoint))
+    capi.gradient(g, d["coef"], s, _dev(capi, d_obs), grad, 0, capi.HSX_CKPT_FP32)
+    gref, _ = oracle.gradient(sh["o"], sh["coef"], sh["iv"], sh["dt"], sh["nt"], sh["src"], sh["wavelet"], sh["rec"],
+                              d_obs)
+    x = grad.download(np.float64)
+    assert np.all(x[~sh["grad_mask"]] == 0)
+    _same(x, gref)
+
+
+def test_indices_outside_the_grid_are_refused_on_the_host(capi):
+    p = make_layout(3, (12, 11, 10), 4)
+    capi.check_indices([0, p.npoint - 1], p.npoint)
+    for bad in ([-1], [p.npoint], [3, p.npoint + 5]):
+        with pytest.raises(ValueError):
+            capi.check_indices(bad, p.npoint)

@@ -62,3 +62,32 @@ def test_directional_derivative_matches_central_differences(oracle, capi, torch_
     adj = float(torch.dot(m.grad, dm))
     print(f"adjoint {adj:.9e} central difference {fd:.9e}", flush=True)
     assert adj == pytest.approx(fd, rel=1e-6)
+
+
+@pytest.mark.parametrize("scale,weighted", [(1e-6, False), (1.0, True), (3e-7, True)])
+def test_backward_of_scaled_and_weighted_losses(oracle, capi, torch_cuda, scale, weighted):
+    """losses other than the plain misfit: a tiny scale and per-sample weights.
+    The trace gradient reaches the adjoint as the residual itself, so the
+    gradient is the oracle's for that residual (FP32 grids)"""
+    torch = torch_cuda
+    from hpcscan_amd.autograd import Ac2Shot, modeling
+
+    sh = make_shot(oracle, (26, 22, 20), 8, nt=50, fp64=False, seed=33)
+    shot = Ac2Shot(sh["p"], sh["dt"], sh["src"], sh["wavelet"], sh["rec"])
+    m = torch.tensor(sh["m"], dtype=torch.float64, device="cuda", requires_grad=True)
+    d_obs = torch.tensor(sh["d_obs"], dtype=torch.float32, device="cuda")
+    w = torch.tensor(np.random.default_rng(3).uniform(0.1, 2.0, sh["d_obs"].shape) if weighted
+                     else np.ones(sh["d_obs"].shape), dtype=torch.float64, device="cuda")
+    tr = modeling(m, shot)
+    loss = scale * 0.5 * (w * (tr.double() - d_obs.double()) ** 2).sum()
+    loss.backward()
+    # the residual autograd hands to the adjoint: d loss / d traces, in FP32
+    tr_h = tr.detach().double().cpu().numpy()
+    res = (scale * w.cpu().numpy() * (tr_h - sh["d_obs"].astype(np.float64))).astype(np.float32)
+    gref, _ = oracle.gradient(sh["o"], sh["coef"], sh["iv"], sh["dt"], sh["nt"], sh["src"], sh["wavelet"],
+                              sh["rec"], res, residual=True)
+    got = m.grad.detach().cpu().numpy()
+    err = np.linalg.norm(got - gref) / np.linalg.norm(gref)
+    print(f"scale={scale} weighted={weighted}: rel err {err:.3e}", flush=True)
+    assert np.linalg.norm(gref) > 0
+    assert err < 1e-5

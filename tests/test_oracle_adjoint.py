@@ -68,3 +68,43 @@ def test_fp32_gradient_close_to_fp64(oracle):
                              sh32["wavelet"], sh32["rec"], sh32["d_obs"])
     err = np.linalg.norm(g32 - g64) / np.linalg.norm(g64)
     assert err < 1e-4, err
+
+
+@pytest.mark.parametrize("fp64", [True, False])
+def test_given_residual_equals_formed_residual(oracle, fp64):
+    """the gradient of a residual passed in (the adjoint source of any loss)
+    equals the one formed from the traces and the data, bit for bit"""
+    sh = make_shot(oracle, (14, 13, 12), 8, nt=40, fp64=fp64, seed=13)
+    o, iv, dt = sh["o"], sh["iv"], sh["dt"]
+    args = (o, sh["coef"], iv, dt, sh["nt"], sh["src"], sh["wavelet"], sh["rec"])
+    g, J = oracle.gradient(*args, sh["d_obs"])
+    d = oracle.forward(o, sh["coef"], iv, sh["nt"], sh["src"], sh["wavelet"], sh["rec"])
+    r = (d - sh["d_obs"]).astype(sh["dtype"])
+    g_r, J_r = oracle.gradient(*args, r, residual=True)
+    assert J_r == J
+    assert np.array_equal(g_r, g)
+    # and it is linear in that residual
+    g_s, _ = oracle.gradient(*args, (r * 0.25).astype(sh["dtype"]), residual=True)
+    np.testing.assert_allclose(g_s, 0.25 * g, rtol=1e-5 if not fp64 else 1e-12, atol=1e-12 * np.abs(g).max())
+
+
+def test_source_on_a_boundary_plane(oracle):
+    """a source on the n1 boundary plane: the gradient stays zero off the points
+    the free surface does not touch and still matches central differences"""
+    sh = make_shot(oracle, (16, 14, 12), 8, nt=50, fp64=True, seed=17)
+    p, o, iv, dt = sh["p"], sh["o"], sh["iv"], sh["dt"]
+    r = p.inner()
+    mid2, mid3 = (r[2] + r[3]) // 2, (r[4] + r[5]) // 2
+    sh["src"] = np.array([r[0] + mid2 * p.n1 + mid3 * p.n1 * p.n2, (r[0] + 3) + (mid2 + 1) * p.n1 + mid3 * p.n1 * p.n2],
+                         np.int64)
+    sh["wavelet"] = np.repeat(sh["wavelet"], 2, axis=1).copy()
+    d_obs = oracle.forward(o, sh["coef_true"], iv, sh["nt"], sh["src"], sh["wavelet"], sh["rec"])
+    g, J = oracle.gradient(o, sh["coef"], iv, dt, sh["nt"], sh["src"], sh["wavelet"], sh["rec"], d_obs)
+    assert J > 0
+    assert np.all(g[~sh["grad_mask"]] == 0)
+    rng = np.random.default_rng(5)
+    dm = np.where(sh["inner_mask"], rng.standard_normal(o.npoint), 0.0) * sh["m"] * 0.05
+    eps = 1e-4
+    Jp = _misfit(oracle, p, o, np.where(sh["inner_mask"], dt * dt / (sh["m"] + eps * dm), 0.0), iv, sh, d_obs)
+    Jm = _misfit(oracle, p, o, np.where(sh["inner_mask"], dt * dt / (sh["m"] - eps * dm), 0.0), iv, sh, d_obs)
+    assert float(np.dot(g, dm)) == pytest.approx((Jp - Jm) / (2 * eps), rel=2e-6)
