@@ -61,10 +61,6 @@ Config::Config(void)
     tmax = UNSPECIFIED;
     writeGrid = false;
 
-    gpuDevice = 0;
-    hipKernel = "auto";
-    hipFoldBC = false;
-
     memset(hostName, 0, sizeof(hostName));
     memset(userName, 0, sizeof(userName));
     gethostname(hostName, HOST_NAME_MAX);
@@ -76,7 +72,10 @@ Config::Config(void)
 
 namespace {
 
-enum class Kind { On, Off, Int, Float, Str, DebugLvl, Dim, FdOrder };
+// EnvSet / EnvOn / EnvOff: backend settings, passed on as environment
+// variables (target = the variable's name) rather than Config members, so that
+// a backend reads them without fields the reference's Config lacks
+enum class Kind { On, Off, Int, Float, Str, DebugLvl, Dim, FdOrder, EnvSet, EnvOn, EnvOff };
 enum class Check { None, Pos, NonNeg };
 
 struct Flag {
@@ -143,11 +142,12 @@ std::vector<Flag> flagTable(Config &c)
         {"-testMode", K::Str, &c.testMode, " Test mode\t", C::None, "test mode (see enabled modes in the header)"},
         {"-tmax", K::Float, &c.tmax, " Tmax\t\t", C::Pos, "max. time (s) for propagator, overrides nt"},
         {"-writeGrid", K::On, &c.writeGrid, " Write grids on disk", C::None, "write grids on disk"},
-        // MI355X backend options
-        {"-gpuDevice", K::Int, &c.gpuDevice, " GPU device\t", C::NonNeg, "MI355X: HIP device ordinal"},
-        {"-hipKernel", K::Str, &c.hipKernel, " HIP kernel\t", C::None, "MI355X: stencil kernels auto (DEFAULT) | ref | tiled"},
-        {"-hipFoldBC", K::On, &c.hipFoldBC, " HIP fold BC\t", C::None, "MI355X: fold the free surface into the fused step"},
-        {"-hipNoFoldBC", K::Off, &c.hipFoldBC, " HIP fold BC\t", C::None, "MI355X: apply the free surface as a separate pass (DEFAULT)"},
+        // MI355X backend settings (environment variables HPCSCAN_MI355X_*)
+        {"-gpuDevice", K::EnvSet, (void *)"HPCSCAN_MI355X_DEVICE", " GPU device\t", C::NonNeg, "MI355X: HIP device ordinal"},
+        {"-hipKernel", K::EnvSet, (void *)"HPCSCAN_MI355X_KERNEL", " HIP kernel\t", C::None, "MI355X: stencil kernels auto (DEFAULT) | ref | tiled"},
+        {"-hipFoldBC", K::EnvOn, (void *)"HPCSCAN_MI355X_FOLD_BC", " HIP fold BC\t", C::None, "MI355X: fold the free surface into the fused step"},
+        {"-hipNoFoldBC", K::EnvOff, (void *)"HPCSCAN_MI355X_FOLD_BC", " HIP fold BC\t", C::None, "MI355X: apply the free surface as a separate pass (DEFAULT)"},
+        {"-hipAutotune", K::EnvOn, (void *)"HPCSCAN_MI355X_AUTOTUNE", " HIP autotune\t", C::None, "MI355X: time the tile shapes once per grid layout, keep the fastest"},
     };
 }
 
@@ -159,7 +159,7 @@ void printHelp(const std::vector<Flag> &table)
         switch (f.kind) {
         case Kind::Int: arg = " <int>"; break;
         case Kind::Float: arg = " <float>"; break;
-        case Kind::Str: arg = " <string>"; break;
+        case Kind::Str: case Kind::EnvSet: arg = " <string>"; break;
         case Kind::DebugLvl: arg = " <OPT>"; break;
         case Kind::Dim: case Kind::FdOrder: arg = " <int>"; break;
         default: break;
@@ -209,6 +209,11 @@ Rtn_code Config::parse_argument(int argc, char *argv[])
             printInfo(MASTER, f->label, f->kind == Kind::On ? "YES" : "NO");
             continue;
         }
+        if (f->kind == Kind::EnvOn || f->kind == Kind::EnvOff) {
+            setenv((const char *)f->target, f->kind == Kind::EnvOn ? "1" : "0", 1);
+            printInfo(MASTER, f->label, f->kind == Kind::EnvOn ? "YES" : "NO");
+            continue;
+        }
         if (++ii >= argc) {
             printError((string(" parameter is needed after ") + f->name).c_str());
             return RTN_CODE_KO;
@@ -238,6 +243,14 @@ Rtn_code Config::parse_argument(int argc, char *argv[])
         }
         case Kind::Str:
             *(string *)f->target = v;
+            printInfo(MASTER, f->label, string(v));
+            break;
+        case Kind::EnvSet:
+            if (f->check == Check::NonNeg && atoi(v) < 0) {
+                printError((string(" ") + (f->name + 1) + " should be >= 0").c_str());
+                return RTN_CODE_KO;
+            }
+            setenv((const char *)f->target, v, 1);
             printInfo(MASTER, f->label, string(v));
             break;
         case Kind::DebugLvl: {

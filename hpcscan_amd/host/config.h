@@ -57,9 +57,6 @@ public:
     char userName[LOGIN_NAME_MAX + 1];
 
     // MI355X backend options (no reference counterpart)
-    Myint gpuDevice;        // -gpuDevice <int>  HIP device ordinal
-    string hipKernel;       // -hipKernel auto|ref|tiled  stencil kernel family
-    bool hipFoldBC;         // -hipFoldBC / -hipNoFoldBC  fold the free surface into the fused step
 
 private:
     Config(void);

@@ -23,7 +23,16 @@ DataAcquisition::DataAcquisition(void)
     nt = 0;
     nSrc = 0;
     nRec = nRecLoc = 0;
+    nSampleInTraceLoc = total_nSampleInTrace = 0;
     record_trace_time = trace_write_time = trace_gather_time = total_timeWriteTrace = 0.0;
+}
+
+DataAcquisition::~DataAcquisition(void)
+{
+    delete[] srcIdx;
+    delete[] tab_recIdxLoc;
+    delete[] tab_idxglob;
+    delete[] trace;
 }
 
 // Built-in geometry (src/data_Acquisition.cpp:72-279): one source one point
@@ -48,44 +57,44 @@ Rtn_code DataAcquisition::initialize(DataAcqui_type acquiType, Grid &coefGrid, M
     };
 
     // source
-    srcIdx.assign(1, flat(1, N2 / 2, N3 / 2));
+    delete[] srcIdx;
+    srcIdx = new Myint64[1];
+    srcIdx[0] = flat(1, N2 / 2, N3 / 2);
     nSrc = 1;
     printInfo(MASTER, " Number of sources", nSrc);
 
     // receivers
-    recIdx.clear();
-    recIdxGlob.clear();
+    std::vector<Myint64> rec;
     if (coefGrid.dim == DIM1) {
-        nRec = 1;
-        recIdx.push_back(flat(1, 0, 0));
-        recIdxGlob.push_back(0);
+        rec.push_back(flat(1, 0, 0));
     } else if (coefGrid.dim == DIM2) {
-        nRec = N2;
-        for (Myint64 r2 = 0; r2 < N2; r2++) {
-            recIdx.push_back(flat(1, r2, 0));
-            recIdxGlob.push_back((Myint)r2);
-        }
+        for (Myint64 r2 = 0; r2 < N2; r2++) rec.push_back(flat(1, r2, 0));
     } else {
-        const Myint64 n3Rec = (N3 - 1) / 10 + 1;
-        nRec = N2 * n3Rec;
-        Myint iRec = 0;
         for (Myint64 r3 = 0; r3 < N3; r3 += 10)
-            for (Myint64 r2 = 0; r2 < N2; r2++) {
-                recIdx.push_back(flat(1, r2, r3));
-                recIdxGlob.push_back(iRec++);
-            }
+            for (Myint64 r2 = 0; r2 < N2; r2++) rec.push_back(flat(1, r2, r3));
     }
-    nRecLoc = (Myint)recIdx.size();
+    nRec = nRecLoc = (Myint)rec.size();
+    delete[] tab_recIdxLoc;
+    delete[] tab_idxglob;
+    tab_recIdxLoc = new Myint64[rec.size() ? rec.size() : 1];
+    tab_idxglob = new Myint[rec.size() ? rec.size() : 1];
+    for (size_t i = 0; i < rec.size(); i++) {
+        tab_recIdxLoc[i] = rec[i];
+        tab_idxglob[i] = (Myint)i;
+    }
     printInfo(MASTER, " Number of receivers", nRec);
     printInfo(ALL, " Number of receivers locally", nRecLoc);
-    printInfo(MASTER, " Trace file size (MB)", (Myint64)nRecLoc * nt * sizeof(Myfloat) / 1e6);
-    trace.assign((size_t)nRecLoc * nt, ZERO);
+    nSampleInTraceLoc = total_nSampleInTrace = (Myint64)nRecLoc * nt;
+    printInfo(MASTER, " Trace file size (MB)", total_nSampleInTrace * sizeof(Myfloat) / 1e6);
+    delete[] trace;
+    trace = new Myfloat[nSampleInTraceLoc ? nSampleInTraceLoc : 1];
+    for (Myint64 i = 0; i < nSampleInTraceLoc; i++) trace[i] = ZERO;
     return RTN_CODE_OK;
 }
 
 // Ricker wavelet of dominant frequency freqMax / 2.5, delayed by 1.5 sqrt(6) / (pi f)
 // (src/data_Acquisition.cpp:292-301)
-Myfloat DataAcquisition::rickerTerm(Myint it, Myfloat dt)
+static Myfloat rickerTerm(Myint it, Myfloat dt)
 {
     const Myfloat64 rickerFreq = Config::Instance()->freqMax / 2.5;
     const Myfloat64 da = M_PI * rickerFreq;
@@ -111,7 +120,7 @@ Rtn_code DataAcquisition::recordTrace(Grid &grid, Modeling_type modType, Myint i
 {
     const double t0 = hpcWtime();
     if (modType == FORWARD) {
-        for (Myint iRec = 0; iRec < nRecLoc; iRec++) trace[it + (size_t)nt * iRec] = grid.grid_3d[recIdx[iRec]];
+        for (Myint iRec = 0; iRec < nRecLoc; iRec++) trace[it + (size_t)nt * iRec] = grid.grid_3d[tab_recIdxLoc[iRec]];
     } else if (modType != ADJOINT) {
         printError("In DataAcquisition::recordTrace, unsupported modType", (Myint)modType);
         return RTN_CODE_KO;
@@ -129,7 +138,7 @@ Rtn_code DataAcquisition::writeTrace(string fileName)
     printInfo(MASTER, "* write on disk\t", fbin.c_str());
     std::vector<Myfloat> all((size_t)nRec * nt, 0.0);
     for (Myint i = 0; i < nRecLoc; i++)
-        memcpy(&all[(size_t)recIdxGlob[i] * nt], &trace[(size_t)i * nt], nt * sizeof(Myfloat));
+        memcpy(&all[(size_t)tab_idxglob[i] * nt], &trace[(size_t)i * nt], nt * sizeof(Myfloat));
     const double t0w = hpcWtime();
     const int fd = open(fbin.c_str(), O_WRONLY | O_APPEND | O_CREAT, S_IRUSR | S_IWUSR);
     if (fd == -1) {
@@ -162,7 +171,7 @@ Rtn_code DataAcquisition::writeTrace(string fileName)
 shared_ptr<DataAcquisition> DataAcquisition_Factory::create(string gridMode)
 {
     const GridBackend *b = Grid_Factory::find(gridMode);
-    if (b && b->acquisition) return shared_ptr<DataAcquisition>(b->acquisition());
+    if (b && b->acquisition) return b->acquisition();
     return make_shared<DataAcquisition>();
 }
 

@@ -5,8 +5,20 @@
 // completes before it returns) or writeTrace's copy orders them, so the time
 // loop pays no host round trip for them (400^3 order 8 modeling: the time loop
 // ran at 163.9 GPoint/s when each of them waited for the stream).
+//
+// The class builds against the reference's DataAcquisition
+// (src/data_Acquisition.h:20-80: its member names, its virtuals).  There
+// appliSourceTerm and the destructor are not virtual: a call through a
+// DataAcquisition& would reach the host version, which writes the source into
+// the host array while the wavefield lives in HBM.  initialize() therefore
+// checks that appliSourceTerm reaches this class through a base reference and
+// fails with a message naming the one-word fix otherwise; the factory helper
+// makes the shared pointer from this class, so the destructor runs.
+#include <cmath>
+#include <memory>
 #include <vector>
 
+#include "config.h"
 #include "data_Acquisition.h"
 #include "global.h"
 #include "grid_Mi355x.h"
@@ -17,14 +29,37 @@ namespace hpcscan {
 class DataAcquisition_Mi355x : public DataAcquisition
 {
 public:
-    ~DataAcquisition_Mi355x(void) override
+    ~DataAcquisition_Mi355x(void)
     {
         for (void *p : {d_src, d_wavelet, d_rec, d_traces})
             if (p) hsx_free(p);
     }
 
-    Rtn_code appliSourceTerm(Grid &grid, Modeling_type modType, Myint it, Myfloat dt) override
+    Rtn_code initialize(DataAcqui_type acquiType, Grid &coefGrid, Myint ntIn) override
     {
+        const Rtn_code rc = DataAcquisition::initialize(acquiType, coefGrid, ntIn);
+        if (rc != RTN_CODE_OK) return rc;
+        // does appliSourceTerm dispatch to this class through a base reference?
+        // (an ADJOINT call with a step number no caller uses: the base version
+        // does nothing for ADJOINT)
+        probed = false;
+        DataAcquisition &base = *this;
+        base.appliSourceTerm(coefGrid, ADJOINT, PROBE_STEP, 0);
+        if (!probed) {
+            printError(" In DataAcquisition_Mi355x::initialize, DataAcquisition::appliSourceTerm is not virtual: "
+                       "the source term would go to the host array; declare it virtual (src/data_Acquisition.h:33)");
+            return RTN_CODE_KO;
+        }
+        return RTN_CODE_OK;
+    }
+
+    // not marked override: the reference declares it non-virtual (see above)
+    Rtn_code appliSourceTerm(Grid &grid, Modeling_type modType, Myint it, Myfloat dt)
+    {
+        if (modType == ADJOINT && it == PROBE_STEP) {
+            probed = true;
+            return RTN_CODE_OK;
+        }
         if (modType != FORWARD) return DataAcquisition::appliSourceTerm(grid, modType, it, dt);
         if (nSrc == 0) return RTN_CODE_OK;
         Grid_Mi355x *g = device(grid);
@@ -74,8 +109,22 @@ public:
     }
 
 private:
+    static constexpr Myint PROBE_STEP = -0x5eed;
+    bool probed = false;
     void *d_src = nullptr, *d_wavelet = nullptr, *d_rec = nullptr, *d_traces = nullptr;
     Myfloat dtReady = 0;
+
+    // the reference's source term of step it: Ricker wavelet of dominant
+    // frequency freqMax / 2.5 delayed by 1.5 sqrt(6) / (pi f), in double, then
+    // Myfloat (src/data_Acquisition.cpp:290-301)
+    static Myfloat ricker(Myint it, Myfloat dt)
+    {
+        const Myfloat64 f = Config::Instance()->freqMax / 2.5;
+        const Myfloat64 t00 = 1.5 * sqrt(6.) / (M_PI * f);
+        const Myfloat64 aa = M_PI * f * (it * dt - t00);
+        const Myfloat64 a2 = pow(aa, 2.);
+        return (1. - 2. * a2) * exp(-a2);
+    }
 
     static bool ok(int rc, const char *what)
     {
@@ -101,7 +150,7 @@ private:
                 *p = nullptr;
             }
         std::vector<Myfloat> wav(nt);
-        for (Myint it = 0; it < nt; it++) wav[it] = rickerTerm(it, dt);
+        for (Myint it = 0; it < nt; it++) wav[it] = ricker(it, dt);
         const size_t nr = (size_t)(nRecLoc > 0 ? nRecLoc : 1);
         if (!ok(hsx_malloc(&d_src, sizeof(int64_t)), "hsx_malloc") ||
             !ok(hsx_malloc(&d_wavelet, (size_t)(nt > 0 ? nt : 1) * sizeof(Myfloat)), "hsx_malloc") ||
@@ -112,13 +161,14 @@ private:
         if (!ok(hsx_memcpy_h2d(d_src, &src, sizeof(int64_t), nullptr), "hsx_memcpy_h2d") ||
             !ok(hsx_memcpy_h2d(d_wavelet, wav.data(), wav.size() * sizeof(Myfloat), nullptr), "hsx_memcpy_h2d") ||
             (nRecLoc > 0 &&
-             !ok(hsx_memcpy_h2d(d_rec, recIdx.data(), recIdx.size() * sizeof(int64_t), nullptr), "hsx_memcpy_h2d")))
+             !ok(hsx_memcpy_h2d(d_rec, tab_recIdxLoc, (size_t)nRecLoc * sizeof(int64_t), nullptr), "hsx_memcpy_h2d")))
             return false;
         dtReady = dt;
         return true;
     }
 };
 
-DataAcquisition *newDataAcquisition_Mi355x(void) { return new DataAcquisition_Mi355x(); }
+// for the factory: the shared pointer deletes a DataAcquisition_Mi355x
+shared_ptr<DataAcquisition> makeDataAcquisition_Mi355x(void) { return make_shared<DataAcquisition_Mi355x>(); }
 
 } // namespace hpcscan

@@ -46,7 +46,7 @@ shared_ptr<Grid> Grid_Factory::create(string gridMode, Grid_type gridType)
         printError("IN Grid_Factory::create, return nullptr", gridMode);
         return nullptr;
     }
-    return shared_ptr<Grid>(b->create(gridType));
+    return b->create(gridType);
 }
 
 shared_ptr<Grid> Grid_Factory::create(string gridMode, Grid_type gridType, Dim_type dim, Myint64 n1,
@@ -58,7 +58,7 @@ shared_ptr<Grid> Grid_Factory::create(string gridMode, Grid_type gridType, Dim_t
         printError("IN Grid_Factory::create, return nullptr", gridMode);
         return nullptr;
     }
-    return shared_ptr<Grid>(b->createSized(gridType, dim, n1, n2, n3));
+    return b->createSized(gridType, dim, n1, n2, n3);
 }
 
 shared_ptr<Hardware> Hardware_Factory::create(string gridMode)
@@ -69,8 +69,7 @@ shared_ptr<Hardware> Hardware_Factory::create(string gridMode)
         printError("IN Hardware_Factory::create, return nullptr", gridMode);
         return nullptr;
     }
-    Hardware *h = b->hardware ? b->hardware(gridMode) : new Hardware(gridMode);
-    return h ? shared_ptr<Hardware>(h) : nullptr;
+    return b->hardware ? b->hardware(gridMode) : make_shared<Hardware>(gridMode);
 }
 
 } // namespace hpcscan

@@ -21,13 +21,17 @@ namespace hpcscan {
 
 class DataAcquisition;
 
+// The creators return shared pointers made from the derived type, so that the
+// backend's destructor runs although ~Grid / ~Hardware / ~DataAcquisition are
+// not virtual in the reference (src/grid.h:47, src/hardware.h:26,
+// src/data_Acquisition.h:27).
 struct GridBackend {
     string mode;
-    std::function<Grid *(Grid_type)> create;
-    std::function<Grid *(Grid_type, Dim_type, Myint64, Myint64, Myint64)> createSized;
-    std::function<Hardware *(string)> hardware;
+    std::function<shared_ptr<Grid>(Grid_type)> create;
+    std::function<shared_ptr<Grid>(Grid_type, Dim_type, Myint64, Myint64, Myint64)> createSized;
+    std::function<shared_ptr<Hardware>(string)> hardware;   // may be empty: the host description
     std::function<void(void)> shutdown;   // release process-wide resources, may be empty
-    std::function<DataAcquisition *(void)> acquisition;   // may be empty: host acquisition
+    std::function<shared_ptr<DataAcquisition>(void)> acquisition;   // may be empty: host acquisition
 };
 
 class Grid_Factory

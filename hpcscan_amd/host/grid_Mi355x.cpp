@@ -9,6 +9,7 @@
 
 #include <cmath>
 #include <cstdlib>
+#include <set>
 #include <vector>
 
 #include "config.h"
@@ -23,19 +24,36 @@ namespace hpcscan {
 static const int HSX_STORAGE = sizeof(Myfloat) == 8 ? HSX_FP64 : HSX_FP32;
 static const int HSX_COMPUTE = sizeof(Myfloat2) == 8 ? HSX_FP64 : HSX_FP32;
 
+const Mi355xSettings &mi355xSettings(void)
+{
+    static const Mi355xSettings s = [] {
+        Mi355xSettings x;
+        auto env = [](const char *k) -> const char * {
+            const char *v = getenv(k);
+            return v && *v ? v : nullptr;
+        };
+        if (const char *v = env("HPCSCAN_MI355X_DEVICE")) x.device = atoi(v);
+        if (const char *v = env("HPCSCAN_MI355X_KERNEL")) x.kernel = v;
+        if (const char *v = env("HPCSCAN_MI355X_FOLD_BC")) x.foldBC = atoi(v) != 0;
+        if (const char *v = env("HPCSCAN_MI355X_AUTOTUNE")) x.autotune = atoi(v) != 0;
+        return x;
+    }();
+    return s;
+}
+
 static bool g_libReady = false;
 
 static int initLibrary(void)
 {
     if (g_libReady) return HSX_OK;
-    Config *c = Config::Instance();
-    int rc = hsx_init(c->gpuDevice);
+    const Mi355xSettings &c = mi355xSettings();
+    int rc = hsx_init(c.device);
     if (rc) return rc;
     int fam = HSX_KERNEL_AUTO;
-    if (c->hipKernel == "ref") fam = HSX_KERNEL_REF;
-    else if (c->hipKernel == "tiled") fam = HSX_KERNEL_TILED;
-    else if (c->hipKernel != "auto") {
-        printError(" -hipKernel should be auto, ref or tiled:", c->hipKernel);
+    if (c.kernel == "ref") fam = HSX_KERNEL_REF;
+    else if (c.kernel == "tiled") fam = HSX_KERNEL_TILED;
+    else if (c.kernel != "auto") {
+        printError(" MI355X stencil kernels should be auto, ref or tiled:", c.kernel);
         return HSX_ERR_INVALID;
     }
     rc = hsx_set_kernel(fam);
@@ -74,6 +92,22 @@ Rtn_code Grid_Mi355x::initializeGrid(void)
     if (rc != RTN_CODE_OK) return rc;
     if (!check(initLibrary(), "hsx_init")) return RTN_CODE_KO;
     if (!check(hsx_malloc(&d_grid, npoint * sizeof(Myfloat)), "hsx_malloc")) return RTN_CODE_KO;
+    if (mi355xSettings().autotune && dim == DIM3) {
+        // once per layout in this process; the library keeps the choice for
+        // the fused step and the Laplacian of every grid of that layout
+        static std::set<std::vector<int64_t>> tuned;
+        const hsx_grid d = desc();
+        std::vector<int64_t> key = {d.fd_order, d.storage, d.compute, d.n1, d.n2, d.n3};
+        key.insert(key.end(), d.inner, d.inner + 6);
+        if (tuned.insert(key).second) {
+            int shape[4] = {0, 0, 0, 0};
+            float ms = 0.f;
+            if (!check(hsx_autotune(&d, 3, shape, &ms), "hsx_autotune")) return RTN_CODE_KO;
+            const string txt = to_string(shape[0]) + " lanes x " + to_string(shape[1]) + " rows, chunk " +
+                               to_string(shape[2]) + ", flags " + to_string(shape[3]) + " (" + to_string(ms) + " ms)";
+            printInfo(MASTER, " MI355X autotuned tile", txt);
+        }
+    }
     return RTN_CODE_OK;
 }
 
@@ -296,7 +330,7 @@ Rtn_code Grid_Mi355x::applyBoundaryCondition(BoundCond_type bc)
     // halos already hold the free surface of the inner points (written by the
     // folded step that produced them): the zeros of the boundary planes are
     // owed to the next step, which reads them as zero
-    if (Config::Instance()->hipFoldBC && dim == DIM3 && mask != 0 && fsHalo == mask) {
+    if (mi355xSettings().foldBC && dim == DIM3 && mask != 0 && fsHalo == mask) {
         fsPending = mask;
         return RTN_CODE_OK;
     }
@@ -409,7 +443,7 @@ Rtn_code Grid_Mi355x::computePressureWithFD(Grid &prc, Grid &coef, Myint fdOrder
     hsx_grid g = desc();
     // free surface of prc (owed, or applied with its halos) folded into the step
     const int mask = pp->fsPending ? pp->fsPending : pp->fsHalo;
-    if (mask && Config::Instance()->hipFoldBC) {
+    if (mask && mi355xSettings().foldBC) {
         if (fsPending && fsPending != mask && !settle()) return RTN_CODE_KO;
         const int fold = HSX_FOLD_HALO | (pp->fsPending ? HSX_FOLD_PRC : 0) | (fsPending ? HSX_FOLD_PRN : 0);
         const int rc = hsx_ac2_step_fs(&g, d_grid, pp->d_grid, pc->d_grid, mask, fold, nullptr);
@@ -518,14 +552,14 @@ Myfloat Grid_Mi355x::getMax(Point_type p)
 }
 
 // ---- registration: -testMode MI355X ------------------------------------------------
-DataAcquisition *newDataAcquisition_Mi355x(void);   // data_Acquisition_Mi355x.cpp
+shared_ptr<DataAcquisition> makeDataAcquisition_Mi355x(void);   // data_Acquisition_Mi355x.cpp
 
 static bool registered = Grid_Factory::registerBackend(
-    {GRID_MODE_MI355X, [](Grid_type t) -> Grid * { return new Grid_Mi355x(t); },
-     [](Grid_type t, Dim_type d, Myint64 a, Myint64 b, Myint64 c) -> Grid * {
-         return new Grid_Mi355x(t, d, a, b, c);
+    {GRID_MODE_MI355X, [](Grid_type t) -> shared_ptr<Grid> { return make_shared<Grid_Mi355x>(t); },
+     [](Grid_type t, Dim_type d, Myint64 a, Myint64 b, Myint64 c) -> shared_ptr<Grid> {
+         return make_shared<Grid_Mi355x>(t, d, a, b, c);
      },
-     [](string mode) -> Hardware * { return new Hardware_Mi355x(mode); }, nullptr,
-     [] { return newDataAcquisition_Mi355x(); }});
+     [](string mode) -> shared_ptr<Hardware> { return make_shared<Hardware_Mi355x>(mode); }, nullptr,
+     [] { return makeDataAcquisition_Mi355x(); }});
 
 } // namespace hpcscan

@@ -13,12 +13,30 @@
 
 namespace hpcscan {
 
+// Settings of the MI355X backend, read once from the environment, so that the
+// backend needs no member beyond the reference's Config (src/config.h):
+//   HPCSCAN_MI355X_DEVICE    HIP device ordinal (default 0)
+//   HPCSCAN_MI355X_KERNEL    stencil kernels: auto (default) | ref | tiled
+//   HPCSCAN_MI355X_FOLD_BC   1: fold the free surface into the fused step (default 0)
+//   HPCSCAN_MI355X_AUTOTUNE  1: time the tile shapes once per 3D grid layout and
+//                            keep the fastest (hsx_autotune; default 0)
+// The harness's -gpuDevice / -hipKernel / -hipFoldBC / -hipAutotune flags set them.
+struct Mi355xSettings {
+    int device = 0;
+    string kernel = "auto";
+    bool foldBC = false;
+    bool autotune = false;
+};
+const Mi355xSettings &mi355xSettings(void);
+
 class Grid_Mi355x : public Grid
 {
 public:
     Grid_Mi355x(Grid_type);
     Grid_Mi355x(Grid_type, Dim_type, Myint64, Myint64, Myint64);
-    ~Grid_Mi355x(void) override;
+    // not virtual in the reference (src/grid.h:47): the factory makes the
+    // shared pointer from this class, so this runs
+    ~Grid_Mi355x(void);
 
     Rtn_code initializeGrid(void) override;
     void info(void) override;

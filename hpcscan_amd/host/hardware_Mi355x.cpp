@@ -4,7 +4,7 @@
 #include <dirent.h>
 #include <fstream>
 
-#include "config.h"
+#include "grid_Mi355x.h"
 #include "hpcscan_mi355x.h"
 #include "output_report.h"
 
@@ -19,7 +19,7 @@ void Hardware_Mi355x::info(void)
     printInfo(MASTER, " Target hardware:");
     hostInfo();
     hsx_device_props p;
-    const int dev = Config::Instance()->gpuDevice;
+    const int dev = mi355xSettings().device;
     if (hsx_device_info(dev, &p) == HSX_OK) {
         printInfo(MASTER, " GPU device\t", dev);
         printInfo(MASTER, " GPU name\t", p.name);

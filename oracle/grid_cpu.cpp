@@ -292,16 +292,16 @@ private:
 };
 
 static bool registeredBaseline = Grid_Factory::registerBackend(
-    {GRID_MODE_BASELINE, [](Grid_type t) -> Grid * { return new Grid_CPU(t, 0); },
-     [](Grid_type t, Dim_type d, Myint64 a, Myint64 b, Myint64 c) -> Grid * {
-         return new Grid_CPU(t, d, a, b, c, 0);
+    {GRID_MODE_BASELINE, [](Grid_type t) -> shared_ptr<Grid> { return make_shared<Grid_CPU>(t, 0); },
+     [](Grid_type t, Dim_type d, Myint64 a, Myint64 b, Myint64 c) -> shared_ptr<Grid> {
+         return make_shared<Grid_CPU>(t, d, a, b, c, 0);
      },
      nullptr, nullptr});
 
 static bool registeredCacheBlk = Grid_Factory::registerBackend(
-    {GRID_MODE_CACHEBLK, [](Grid_type t) -> Grid * { return new Grid_CPU(t, 1); },
-     [](Grid_type t, Dim_type d, Myint64 a, Myint64 b, Myint64 c) -> Grid * {
-         return new Grid_CPU(t, d, a, b, c, 1);
+    {GRID_MODE_CACHEBLK, [](Grid_type t) -> shared_ptr<Grid> { return make_shared<Grid_CPU>(t, 1); },
+     [](Grid_type t, Dim_type d, Myint64 a, Myint64 b, Myint64 c) -> shared_ptr<Grid> {
+         return make_shared<Grid_CPU>(t, d, a, b, c, 1);
      },
      nullptr, nullptr});
 

@@ -1,0 +1,404 @@
+"""Parity at the full sizes of the benchmark configurations (run with -m gpu).
+
+* FD_D2 / Propa at 500^3 and 1000^3, orders 4-16, FP32 and FP64:
+  - the kernels bit for bit on windows of n3 planes (first, middle and last
+    inner planes): every output plane of a window depends only on the 2R+1
+    planes around it, so the oracle run on a grid of those planes (same n1 x n2
+    layout, same spacing) must give the device's values exactly;
+  - Ac2SplitComp (Laplacian + updatePressure) equal to Ac2Standard over the
+    whole grid;
+  - the harness (-testMode MI355X) with the reference's command line, mode
+    numbers 2,2,2 and 200 steps: errTestCase against the closed-form discrete
+    eigen-mode recurrence (FP64: to rounding; FP32: within the float rounding
+    of the grid), Standard and SplitComp printing the same error, and at 500^3
+    the same line as the CPU harness (-testMode CacheBlk).  FD_D2: the FP64 L1
+    errors against the discrete symbol, the FP32 ones against the CPU harness.
+* 32 batched shots of 192^3-320^3, order 8, 1000 steps: two shots' traces
+  against the oracle, bit for bit.
+* The 200^3 x 1000-step gradient: central differences of the misfit with the
+  device FP64 forward, and the 16-bit checkpoint gradients against the FP64
+  gradient.
+* One 512^3 order-16 x 2000-step shot: 16-bit checkpoint gradients against the
+  FP64 gradient (relative L2 <= 1e-3 for FP16; the BF16 figure is printed).
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, report_value, run_bin
+from hpcscan_amd import binary
+from hpcscan_amd.layout import make_layout
+
+pytestmark = pytest.mark.gpu
+
+CPU_BIN = {"fp32": os.path.join(ROOT, "oracle", "_build", "hpcscan_cpu"),
+           "fp64": os.path.join(ROOT, "oracle", "_build", "hpcscan_cpu_fp64")}
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.fail("no GPU visible to torch")
+    return torch
+
+
+# ---------------------------------------------------------------------------
+# kernels bit for bit on plane windows of the full grids
+# ---------------------------------------------------------------------------
+WINDOW = 4   # output planes per window
+
+
+def _same(got, ref, what):
+    eq = got == ref
+    if not eq.all():
+        i = np.flatnonzero(~eq)[:4]
+        raise AssertionError(f"{what}: {(~eq).sum()} of {eq.size} values differ, e.g. {got[i]} vs {ref[i]}")
+
+
+@pytest.mark.parametrize("n", [500, 1000])
+@pytest.mark.parametrize("order", [4, 8, 12, 16])
+@pytest.mark.parametrize("fp64", [False, True])
+def test_full_size_kernels_bit_exact_on_plane_windows(oracle, capi, torch_cuda, n, order, fp64):
+    torch = torch_cuda
+    print(f"windows n={n} order={order} fp64={fp64}", flush=True)
+    R = order // 2
+    p = make_layout(3, (n, n, n), order, fp64=fp64)
+    g = capi.hsx_grid(p)
+    r = p.inner()
+    N1, N2 = r[1] - r[0] + 1, r[3] - r[2] + 1
+    win = oracle.layout(3, [N1, N2, WINDOW], order, storage_fp64=fp64)
+    assert (win.n1, win.n2, win.n3) == (p.n1, p.n2, WINDOW + 2 * R)
+    n12 = p.n1 * p.n2
+    npdt = np.float64 if fp64 else np.float32
+    iv = oracle.inv2(p.d, npdt)
+    wr = oracle.index_range(win, oracle.INNER_POINTS)
+    assert wr[4] == R and wr[0:4] == r[0:4]
+    inner = np.zeros((win.n3, win.n2, win.n1), bool)
+    inner[wr[4]:wr[5] + 1, wr[2]:wr[3] + 1, wr[0]:wr[1] + 1] = True
+    inner = inner.ravel()
+    zs = [r[4], (r[4] + r[5]) // 2, r[5] - WINDOW + 1]      # first, middle and last output planes
+
+    def window(t, z0):
+        return t[(z0 - R) * n12:(z0 + WINDOW + R) * n12].cpu().numpy()
+
+    tdt = torch.float64 if fp64 else torch.float32
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(1000 * order + n + (1 if fp64 else 0))
+    u = torch.randn(p.npoint, generator=gen, dtype=tdt, device="cuda")
+    prn = torch.randn(p.npoint, generator=gen, dtype=tdt, device="cuda")
+    coef = torch.rand(p.npoint, generator=gen, dtype=tdt, device="cuda") * 0.1
+    w = torch.zeros_like(u)
+    torch.cuda.synchronize()
+
+    # FD_D2 along each axis and the Laplacian (src/grid.cpp:3232-4577)
+    for name, run, ref_fn in (
+            ("D2 n1", lambda: capi.fd_d2(g, 0, u, w), lambda a, o: oracle.fd_d2(win, 0, a, o, iv)),
+            ("D2 n2", lambda: capi.fd_d2(g, 1, u, w), lambda a, o: oracle.fd_d2(win, 1, a, o, iv)),
+            ("D2 n3", lambda: capi.fd_d2(g, 2, u, w), lambda a, o: oracle.fd_d2(win, 2, a, o, iv)),
+            ("Laplacian", lambda: capi.laplacian(g, u, w), lambda a, o: oracle.laplacian(win, a, o, iv))):
+        run()
+        capi.sync()
+        for z0 in zs:
+            ref = np.zeros(win.npoint, npdt)
+            ref_fn(window(u, z0), ref)
+            _same(window(w, z0)[inner], ref[inner], f"{name} planes {z0}..{z0 + WINDOW - 1}")
+
+    # the propagator step: free surface on prc, then the fused update of prn
+    # (src/propagator_Ac2.cpp:366-393; src/grid.cpp:4612-5112)
+    capi.free_surface(g, u)
+    capi.sync()
+    prn0 = prn.clone()
+    torch.cuda.synchronize()
+    capi.ac2_step(g, prn, u, coef)
+    capi.sync()
+    for z0 in zs:
+        pw = window(prn0, z0).copy()
+        oracle.pressure_fd(win, pw, window(u, z0), window(coef, z0), iv)
+        _same(window(prn, z0)[inner], pw[inner], f"Ac2 step planes {z0}..{z0 + WINDOW - 1}")
+
+    # Ac2SplitComp: Laplacian into lapla, then updatePressure -- the same grid
+    # as the fused step, bit for bit, everywhere
+    capi.laplacian(g, u, w)
+    capi.update_pressure(g, prn0, u, coef, w)
+    capi.sync()
+    assert torch.equal(prn0, prn)
+
+
+# ---------------------------------------------------------------------------
+# the harness at full size
+# ---------------------------------------------------------------------------
+def _propa_args(n, order, prop):
+    return ["-testCase", "Propa", "-propagator", prop, "-dim", "3", "-n1", str(n), "-n2", str(n), "-n3", str(n),
+            "-fdOrder", str(order), "-nt", "200", "-param1", "2", "-param2", "2", "-param3", "2", "-ntry", "1"]
+
+
+@pytest.mark.parametrize("n", [500, 1000])
+@pytest.mark.parametrize("order", [4, 8, 12, 16])
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_propa_full_size_against_closed_form(tmp_path, oracle, n, order, prec):
+    """Propa Ac2Standard and Ac2SplitComp, modes 2,2,2, 200 steps: the printed
+    errTestCase against the closed-form discrete eigen-mode recurrence, the
+    same for both propagators"""
+    errs = {}
+    for prop in ("Ac2Standard", "Ac2SplitComp"):
+        print(f"propa n={n} order={order} {prec} {prop}", flush=True)
+        rc, out = run_bin(binary(prec), _propa_args(n, order, prop) + ["-testMode", "MI355X"], tmp_path, timeout=600)
+        assert rc == 0, out[-2000:]
+        assert report_value(out, "No. time steps") == "200"
+        assert ">> Test FAILED" not in out and "F A T A L" not in out
+        errs[prop] = report_value(out, "errTestCase2")
+        gps = report_value(out, "Best Gpoint/s eff.")
+        print(f"  errTestCase2 {errs[prop]}  Best Gpoint/s eff. {gps}", flush=True)
+    assert errs["Ac2Standard"] == errs["Ac2SplitComp"]
+    err = float(errs["Ac2Standard"])
+    cf, prm = oracle.eigen_closed_form([n] * 3, order, (2, 2, 2), fp64=prec == "fp64", nt_cfg=200)
+    assert prm["nt"] == 200
+    rel = abs(err / cf - 1)
+    print(f"  closed form {cf:.6e}, relative gap {rel:.2e}", flush=True)
+    # FP64: the grid's rounding is far below the mode's error; FP32: the
+    # recurrence carries the float weights, dt and 1/h^2, what is left is the
+    # float rounding of the 10^8-10^9 grid values (a few 1e-7 absolute)
+    if prec == "fp64":
+        assert rel < 1e-5
+    else:
+        assert abs(err - cf) < 2e-6 + 5e-3 * cf
+
+
+@pytest.mark.parametrize("order", [4, 8, 12, 16])
+def test_propa_500_fp32_same_line_as_cpu(tmp_path, order):
+    """500^3 FP32 Ac2Standard, 200 steps: errTestCase2 printed by the MI355X
+    harness equals the CPU harness's (the kernels are bit-exact; the double
+    sums differ in order only)"""
+    args = _propa_args(500, order, "Ac2Standard")
+    rc1, gpu = run_bin(binary("fp32"), args + ["-testMode", "MI355X"], tmp_path, timeout=600)
+    rc2, cpu = run_bin(CPU_BIN["fp32"], args + ["-testMode", "CacheBlk"], tmp_path, timeout=600)
+    assert rc1 == 0 and rc2 == 0
+    print(f"order {order}: GPU {report_value(gpu, 'errTestCase2')} CPU {report_value(cpu, 'errTestCase2')}; "
+          f"GPU {report_value(gpu, 'Best Gpoint/s eff.')} CPU {report_value(cpu, 'Best Gpoint/s eff.')} GPoint/s",
+          flush=True)
+    assert report_value(gpu, "errTestCase2") == report_value(cpu, "errTestCase2")
+
+
+def _fd_d2_args(n, order):
+    return ["-testCase", "FD_D2", "-dim", "3", "-n1", str(n), "-n2", str(n), "-n3", str(n), "-fdOrder", str(order),
+            "-param1", "2", "-param2", "2", "-param3", "2", "-ntry", "1"]
+
+
+def _l1_lines(out):
+    return [float(l.split()[-1]) for l in out.splitlines() if "L1 err." in l]
+
+
+def _fd_d2_closed_form(oracle, n, order, fp64):
+    """expected L1 errors of the four FD_D2 cases (n1, n2, n3, Laplacian) in
+    exact arithmetic: |S(ah)/h^2 + a^2| / a^2 per axis, with a = 2 pi, the
+    Myfloat2 weights and 1/h^2 (src/testCase_FD_D2.cpp:65-96)"""
+    T = (lambda x: float(x)) if fp64 else (lambda x: float(np.float32(x)))
+    A = [T(a) for a in oracle.d2_coefs(order)]
+    h = 1.0 / (n - 1)
+    inv = T(1.0 / h)
+    inv2 = T(inv * inv)
+    a = oracle.PI_REF * 2
+    S = (A[0] + 2 * sum(A[j] * math.cos(j * a * h) for j in range(1, len(A)))) * inv2
+    axis = abs(S + a * a) / (a * a)
+    return [axis] * 3 + [abs(3 * S + 3 * a * a) / (3 * a * a)]
+
+
+@pytest.mark.parametrize("n", [500, 1000])
+@pytest.mark.parametrize("order", [4, 8, 12, 16])
+def test_fd_d2_full_size_fp64_against_symbol(tmp_path, oracle, n, order):
+    rc, out = run_bin(binary("fp64"), _fd_d2_args(n, order) + ["-testMode", "MI355X"], tmp_path, timeout=600)
+    assert rc == 0, out[-2000:]
+    l1 = _l1_lines(out)
+    cf = _fd_d2_closed_form(oracle, n, order, True)
+    print(f"FD_D2 {n}^3 O{order} FP64: L1 {l1} closed form {cf}", flush=True)
+    assert len(l1) == 4
+    for got, ref in zip(l1, cf):
+        # double rounding of a second difference: ~eps * sum|A| / (a h)^2
+        assert abs(got - ref) < 1e-9 + 1e-6 * ref
+    assert ">> Test FAILED" not in out
+
+
+@pytest.mark.parametrize("order", [4, 8, 12, 16])
+def test_fd_d2_500_fp32_same_lines_as_cpu(tmp_path, order):
+    """FP32 at 500^3: the L1 errors are the float rounding of the second
+    differences (above the test case's 1e-3 bar, as on the CPU); the MI355X
+    run prints the CPU run's values"""
+    args = _fd_d2_args(500, order)
+    rc1, gpu = run_bin(binary("fp32"), args + ["-testMode", "MI355X"], tmp_path, timeout=600)
+    rc2, cpu = run_bin(CPU_BIN["fp32"], args + ["-testMode", "CacheBlk"], tmp_path, timeout=600)
+    assert rc1 == 0 and rc2 == 0
+    lg, lc = _l1_lines(gpu), _l1_lines(cpu)
+    print(f"FD_D2 500^3 O{order} FP32: GPU {lg} CPU {lc}", flush=True)
+    assert len(lg) == 4 and lg == lc
+
+
+@pytest.mark.parametrize("order", [4, 8, 12, 16])
+def test_fd_d2_1000_fp32_runs(tmp_path, order):
+    """FP32 at 1000^3: the kernels are pinned bit for bit by the plane windows;
+    here the test case runs and its L1 errors stay at the float-rounding level"""
+    rc, out = run_bin(binary("fp32"), _fd_d2_args(1000, order) + ["-testMode", "MI355X"], tmp_path, timeout=600)
+    assert rc == 0, out[-2000:]
+    l1 = _l1_lines(out)
+    print(f"FD_D2 1000^3 O{order} FP32: L1 {l1}; Best Gpoint/s eff. {report_value(out, 'Best Gpoint/s eff.')}",
+          flush=True)
+    assert len(l1) == 4 and all(0 < x < 0.1 for x in l1)
+
+
+# ---------------------------------------------------------------------------
+# batched shots
+# ---------------------------------------------------------------------------
+def test_batch_32_shots_1000_steps(oracle, capi):
+    """32 shots, N^3 with N drawn in [192, 320], layered 1500-4500 m/s x
+    (1 + 0.05 G), 10 Hz Ricker, FP32 order 8, 1000 steps in one batch (HIP
+    graph replay): the two smallest shots' traces equal the oracle's"""
+    import time
+
+    from hpcscan_amd import acquisition as acq
+
+    rng = np.random.default_rng(2024)
+    ns, nt, h, vmin, vmax, f0 = 32, 1000, 10.0, 1500.0, 4500.0, 10.0
+    sizes = [int(4 * round(x / 4)) for x in rng.uniform(192, 320, ns)]
+    dt = float(np.float32(0.8 * acq.fd_cfl(8) * h / vmax))
+    wav = acq.ricker(nt, dt, f0).astype(np.float32).reshape(nt, 1)
+    grids, keep, shots, host = [], [], [], []
+    pts = 0
+    for s, n in enumerate(sizes):
+        p = make_layout(3, (n, n, n), 8)
+        p.d = [h, h, h]
+        v = acq.layered_velocity(p, vmin, vmax, 0.05, seed=100 + s)
+        r = p.inner()
+        inner = np.zeros((p.n3, p.n2, p.n1), bool)
+        inner[r[4]:r[5] + 1, r[2]:r[3] + 1, r[0]:r[1] + 1] = True
+        coef_h = np.where(inner.ravel(), (v * dt) ** 2, 0.0).astype(np.float32)
+        src_h = np.array([acq.source_top_center(p, 1)], np.int64)
+        rec_h = acq.receiver_line(p, None, 1)
+        capi.check_indices(src_h, p.npoint)
+        capi.check_indices(rec_h, p.npoint)
+        coef = capi.DeviceArray.from_numpy(coef_h)
+        src, rec, w = (capi.DeviceArray.from_numpy(x) for x in (src_h, rec_h, wav))
+        tr = capi.DeviceArray(nt * len(rec_h) * 4)
+        shots.append(capi.make_shot(nt, dt, (src, 1), w, (rec, len(rec_h))))
+        grids.append(capi.hsx_grid(p))
+        keep.append((coef, src, rec, w, tr))
+        host.append((p, coef_h, src_h, rec_h))
+        pts += n ** 3
+    b = capi.Batch(grids, [k[0] for k in keep], shots, [k[4] for k in keep])
+    t0 = time.perf_counter()
+    b.run(use_graph=True)
+    t = time.perf_counter() - t0
+    print(f"batch: {ns} shots, N in [{min(sizes)}, {max(sizes)}], {pts / 1e9:.3f} G points x {nt} steps in "
+          f"{t:.3f} s (first replay, capture included): {pts * nt / t / 1e9:.1f} GPoint/s", flush=True)
+    b.close()
+    for s in sorted(range(ns), key=lambda k: sizes[k])[:2]:
+        p, coef_h, src_h, rec_h = host[s]
+        o = oracle.layout(3, [sizes[s]] * 3, 8)
+        assert o.npoint == p.npoint
+        ref = oracle.forward(o, coef_h, oracle.inv2(p.d, np.float32), nt, src_h, wav, rec_h)
+        got = keep[s][4].download(np.float32).reshape(ref.shape)
+        assert np.abs(ref).max() > 0
+        _same(got.ravel(), ref.ravel(), f"shot {s} ({sizes[s]}^3) traces")
+
+
+# ---------------------------------------------------------------------------
+# the adjoint at its benchmark sizes
+# ---------------------------------------------------------------------------
+def _adjoint_setup(capi, n, order, nt, seed=4):
+    """n^3, h 10 m, smooth velocity 2000-4000 m/s, 8 Hz Ricker one point below
+    the top centre, 200 receivers on a line one point below the top, data from
+    a +-3 % smoothly perturbed model (the FP64 forward), dt 0.8 x CFL"""
+    from hpcscan_amd import acquisition as acq
+
+    h, vmin, vmax, f0 = 10.0, 2000.0, 4000.0, 8.0
+    lay = {}
+    for fp64 in (False, True):
+        p = make_layout(3, (n, n, n), order, fp64=fp64)
+        p.d = [h, h, h]
+        lay[fp64] = p
+    p = lay[False]
+    v = acq.smooth_velocity(p, vmin, vmax, seed=seed)
+    m = 1.0 / (v * v)
+    m_true = m * (1.0 + 0.03 * np.clip(acq.gaussian_field((p.n3, p.n2, p.n1), 44, 16.0).ravel() / 3.0, -1, 1))
+    dt = float(np.float32(0.8 * acq.fd_cfl(order) * h / vmax))
+    r = p.inner()
+    inner = np.zeros((p.n3, p.n2, p.n1), bool)
+    inner[r[4]:r[5] + 1, r[2]:r[3] + 1, r[0]:r[1] + 1] = True
+    inner = inner.ravel()
+    src = np.array([acq.source_top_center(p, 1)], np.int64)
+    rec = acq.receiver_line(p, 200, 1)
+    wav = acq.ricker(nt, dt, f0)
+    S = {}
+    for fp64 in (False, True):
+        dtp = np.float64 if fp64 else np.float32
+        dev = {k: capi.DeviceArray.from_numpy(np.ascontiguousarray(x)) for k, x in
+               (("src", src), ("rec", rec), ("wav", wav.astype(dtp).reshape(nt, 1)))}
+        S[fp64] = dict(g=capi.hsx_grid(lay[fp64]), dtype=dtp, dev=dev,
+                       shot=capi.make_shot(nt, dt, (dev["src"], 1), dev["wav"], (dev["rec"], len(rec))))
+    d64 = capi.DeviceArray(nt * len(rec) * 8)
+    ct = capi.DeviceArray.from_numpy(np.where(inner, dt * dt / m_true, 0.0))
+    capi.forward(S[True]["g"], ct, S[True]["shot"], d64)
+    dobs64 = d64.download(np.float64)
+    return dict(p=p, m=m, dt=dt, inner=inner, S=S, d64=d64, dobs32=capi.DeviceArray.from_numpy(
+        dobs64.astype(np.float32)), dobs64=dobs64, nt=nt, nrec=len(rec))
+
+
+def _grad(capi, A, fp64, every, fmt, dobs):
+    S = A["S"][fp64]
+    coef = capi.DeviceArray.from_numpy(np.where(A["inner"], A["dt"] ** 2 / A["m"], 0.0).astype(S["dtype"]))
+    gr = capi.DeviceArray.from_numpy(np.zeros(A["p"].npoint))
+    plan = capi.AdjointPlan(S["g"], A["nt"], A["nrec"], every, fmt)
+    J = plan.gradient(S["g"], coef, S["shot"], dobs, gr)
+    plan.close()
+    out = gr.download(np.float64)
+    gr.free()
+    coef.free()
+    return out, J
+
+
+def test_adjoint_200_cube_1000_steps(capi):
+    """200^3 order 8, 1000 steps, checkpoints every 20: the FP64 gradient
+    (storage-type checkpoints, exact recomputation) against central
+    differences of the misfit along a smooth perturbation, computed with the
+    device FP64 forward; FP16 / BF16 checkpoint gradients against it"""
+    A = _adjoint_setup(capi, 200, 8, 1000)
+    g64, J64 = _grad(capi, A, True, 20, capi.HSX_CKPT_FP32, A["d64"])
+    p, dt, inner = A["p"], A["dt"], A["inner"]
+    from hpcscan_amd import acquisition as acq
+
+    dm = np.where(inner, acq.gaussian_field((p.n3, p.n2, p.n1), 7, 20.0).ravel(), 0.0) * A["m"] * 0.02
+    S = A["S"][True]
+    tr = capi.DeviceArray(A["nt"] * A["nrec"] * 8)
+
+    def J_of(mm):
+        c = capi.DeviceArray.from_numpy(np.where(inner, dt * dt / mm, 0.0))
+        capi.forward(S["g"], c, S["shot"], tr)
+        r = tr.download(np.float64) - A["dobs64"]
+        return 0.5 * float(np.dot(r, r))
+
+    eps = 1e-3
+    fd = (J_of(A["m"] + eps * dm) - J_of(A["m"] - eps * dm)) / (2 * eps)
+    adj = float(np.dot(g64, dm))
+    print(f"200^3 x 1000: J {J64:.6e}; <grad, dm> {adj:.9e}, central difference {fd:.9e}", flush=True)
+    assert adj == pytest.approx(fd, rel=1e-5)
+    nrm = np.linalg.norm(g64)
+    for name, fmt, tol in (("fp16", capi.HSX_CKPT_FP16, 1e-3), ("bf16", capi.HSX_CKPT_BF16, 1e-2)):
+        g, J = _grad(capi, A, False, 20, fmt, A["dobs32"])
+        err = float(np.linalg.norm(g - g64) / nrm)
+        print(f"  {name} checkpoints: relative L2 vs FP64 {err:.3e}, misfit {J:.6e}", flush=True)
+        assert err < tol
+
+
+def test_adjoint_512_cube_order16_2000_steps(capi):
+    """512^3 order 16, 2000 steps: FP16 (every 20 steps) and BF16 checkpoint
+    gradients against the FP64 gradient (checkpoints every 100)"""
+    A = _adjoint_setup(capi, 512, 16, 2000)
+    g64, _ = _grad(capi, A, True, 100, capi.HSX_CKPT_FP32, A["d64"])
+    nrm = np.linalg.norm(g64)
+    assert nrm > 0
+    for name, fmt, tol in (("fp16", capi.HSX_CKPT_FP16, 1e-3), ("bf16", capi.HSX_CKPT_BF16, 2e-2)):
+        g, _ = _grad(capi, A, False, 20, fmt, A["dobs32"])
+        err = float(np.linalg.norm(g - g64) / nrm)
+        print(f"512^3 O16 x 2000, {name} checkpoints every 20: relative L2 vs FP64 {err:.3e}", flush=True)
+        assert err < tol
