@@ -176,3 +176,16 @@ def test_published_1000_cube_error(tmp_path):
     assert report_value(out, "No. time steps") == "451"
     err = float(report_value(out, "errTestCase2"))
     assert abs(err / 0.0340336 - 1) < 5e-5
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_autotune_flag_same_report(tmp_path, prec):
+    """-hipAutotune: the backend tunes the tile shape of the grid layout once
+    (printed), and the errors equal the CPU run's"""
+    args = ["-testCase", "Propa", "-dim", "3", "-n1", "100", "-n2", "96", "-n3", "90", "-fdOrder", "8", "-nt", "30"]
+    rc1, gpu = run_bin(binary(prec), args + ["-testMode", "MI355X", "-ntry", "1", "-hipAutotune"], tmp_path)
+    rc2, cpu = run_bin(CPU_BIN[prec], args + ["-testMode", "CacheBlk", "-ntry", "1"], tmp_path)
+    assert rc1 == 0 and rc2 == 0
+    assert report_value(gpu, "MI355X autotuned tile") is not None, gpu[-3000:]
+    print(report_value(gpu, "MI355X autotuned tile"))
+    assert _errors(gpu) == _errors(cpu)

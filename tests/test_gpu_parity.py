@@ -529,3 +529,48 @@ def test_folded_free_surface_refuses_small_axes(capi):
         capi.ac2_step_fs(g, d[0], d[1], d[2], capi.HSX_FOLD_PRC)
     assert e.value.rc == capi.HSX_ERR_UNSUPPORTED
     capi.ac2_step_fs(g, d[0], d[1], d[2], capi.HSX_FOLD_PRC, face_mask=0x0F)   # n3 faces not folded
+
+
+def _step_ms(capi, g, dp, du, dc, reps):
+    """median HIP-event time of the fused step (library stream)"""
+    ev = [(capi.Event(), capi.Event()) for _ in range(reps)]
+    for e0, e1 in ev:
+        e0.record()
+        capi.ac2_step(g, dp, du, dc)
+        e1.record()
+    capi.sync()
+    return sorted(e0.elapsed_ms(e1) for e0, e1 in ev)[reps // 2]
+
+
+@pytest.mark.parametrize("n,order,prec", [((400, 400, 400), 8, "f32"), ((320, 300, 280), 16, "f32"),
+                                          ((256, 240, 200), 8, "f64")])
+def test_autotuned_shape_bit_exact_and_not_slower(oracle, capi, n, order, prec):
+    """hsx_autotune picks a tile shape for the layout (its sweep includes the
+    library's default shape); the fused step and the Laplacian then run with
+    it, bit for bit as the oracle, in no more time than the default shape"""
+    dtype, c64 = PRECS[prec]
+    p, o = _layouts(oracle, 3, n, order, dtype)
+    g = capi.hsx_grid(p, compute_fp64=c64)
+    rng = np.random.default_rng(order + n[0])
+    u, coef, prn0 = (_rand(rng, p.npoint, dtype) for _ in range(3))
+    du, dc, dp = (capi.DeviceArray.from_numpy(x) for x in (u, coef, prn0))
+    capi.autotune_clear()
+    try:
+        default_ms = _step_ms(capi, g, dp, du, dc, 9)
+        shape, tune_ms = capi.autotune(g, reps=5)
+        tuned_ms = _step_ms(capi, g, dp, du, dc, 9)
+        print(f"n={n} O{order} {prec}: tuned shape {shape} ({tune_ms:.4f} ms in the sweep), "
+              f"median {tuned_ms:.4f} ms against {default_ms:.4f} ms for the default", flush=True)
+        assert tuned_ms <= default_ms * 1.05 + 0.002
+        dp.upload(prn0)
+        ref = prn0.copy()
+        oracle.pressure_fd(o, ref, u, coef, oracle.inv2(p.d, dtype, compute64=c64), compute64=c64)
+        capi.ac2_step(g, dp, du, dc)
+        _same(dp.download(dtype), ref)
+        w = np.zeros(p.npoint, dtype)
+        dw = capi.DeviceArray.from_numpy(w)
+        oracle.laplacian(o, u, w, oracle.inv2(p.d, dtype, compute64=c64), compute64=c64)
+        capi.laplacian(g, du, dw)
+        _same(dw.download(dtype), w)
+    finally:
+        capi.autotune_clear()
