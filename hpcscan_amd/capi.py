@@ -92,6 +92,7 @@ SIGNATURES = {
     "hsx_sum_abs": (_i, [_G, _vp, _R, _P(_d), _vp]),
     "hsx_sum_abs_diff": (_i, [_G, _vp, _vp, _R, _P(_d), _vp]),
     "hsx_l1_sums": (_i, [_G, _vp, _vp, _R, _P(_d), _vp]),
+    "hsx_l1_sums_noalloc": (_i, [_G, _vp, _vp, _R, _vp, _vp]),
     "hsx_max_err": (_i, [_G, _vp, _vp, _R, _d, _P(_d), _vp]),
     "hsx_min_max": (_i, [_G, _vp, _R, _P(_d), _vp]),
     "hsx_ac2_step_to": (_i, [_G, _vp, _vp, _vp, _vp, _vp]),

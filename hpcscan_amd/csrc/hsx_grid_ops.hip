@@ -662,6 +662,35 @@ static int red_blocks(const Box &b)
     return (int)(nb < RED_BLOCKS ? nb : RED_BLOCKS);
 }
 
+// Scratch of hsx_l1_sums_noalloc: the two partial sums of each of at most
+// RED_BLOCKS blocks (16 KiB), one copy per device for the life of the process.
+// Host code never uses this name as an address: it takes the device address
+// of each device's copy from hipGetSymbolAddress (noalloc_scratch below).
+static __device__ __align__(256) double scratch[2 * RED_BLOCKS];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_noalloc_mu;
+static double *g_noalloc_ptr[NOALLOC_MAX_DEVICES];   // device address of scratch, per device
+
+// the current device's copy of scratch, looked up once per device (with that
+// device current)
+static int noalloc_scratch(double **out)
+{
+    int dev = -1;
+    HSX_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+        return fail(HSX_ERR_UNSUPPORTED, "device %d: the allocation-free entry points serve devices 0..%d", dev,
+                    NOALLOC_MAX_DEVICES - 1);
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void *p = nullptr;
+        HSX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(HSX_ERR_HIP, "hipGetSymbolAddress(scratch) gave a null address");
+        g_noalloc_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_noalloc_ptr[dev];
+    return HSX_OK;
+}
+
 template <int OP>
 static int run_reduce(const hsx_grid *g, const void *u, const void *v, const int64_t r[6], float maxerr,
                       double out[2], void *stream)
@@ -885,6 +914,38 @@ int hsx_l1_sums(const hsx_grid *g, const void *u, const void *v, const int64_t r
                 void *stream)
 {
     return run_reduce<RED_L1>(g, u, v, range, 0.f, out, stream);
+}
+
+int hsx_l1_sums_noalloc(const hsx_grid *g, const void *u, const void *v, const int64_t range[6], double *out,
+                        void *stream)
+{
+    int rc = check_grid(g);
+    if (!rc) rc = check_range(g, range);
+    if (!rc) rc = check_ptr(u, "u");
+    if (!rc) rc = check_ptr(v, "v");
+    if (!rc) rc = check_ptr(out, "out");
+    if (!rc) rc = ensure_init();
+    if (rc) return rc;
+    double *part = nullptr;
+    if ((rc = noalloc_scratch(&part))) return rc;
+    const Box b = box_of(range);
+    const int nb = red_blocks(b);
+    if (nb < 1 || nb > RED_BLOCKS)
+        return fail(HSX_ERR_INVALID, "l1_sums_noalloc: %d partial blocks, the scratch holds %d", nb, RED_BLOCKS);
+    hipStream_t s = stream_of(stream);
+    // every block writes its own partials before k_reduce_final reads them:
+    // nothing in scratch has to start at zero
+    rc = dispatch_types(g, [&](auto ts, auto) -> int {
+        using TS = decltype(ts);
+        hipLaunchKernelGGL((k_reduce<TS, RED_L1>), dim3(nb), dim3(ROW_THREADS, ROWS_PER_BLOCK), 0, s, (const TS *)u,
+                           (const TS *)v, b, g->n1, g->n1 * g->n2, 0.f, part);
+        HSX_LAUNCHED("k_reduce");
+        return HSX_OK;
+    });
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_reduce_final<RED_L1>), dim3(1), dim3(256), 0, s, part, nb, out);
+    HSX_LAUNCHED("k_reduce_final");
+    return HSX_OK;
 }
 
 int hsx_max_err(const hsx_grid *g, const void *u, const void *v, const int64_t range[6], double max_err_float,

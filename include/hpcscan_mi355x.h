@@ -189,6 +189,16 @@ int hsx_sum_abs_diff(const hsx_grid *g, const void *u, const void *v, const int6
 /* out[0] = sum |u - v|, out[1] = sum |v| (Grid::L1Err before its divide) */
 int hsx_l1_sums(const hsx_grid *g, const void *u, const void *v, const int64_t range[6],
                 double out[2], void *stream);
+/* hsx_l1_sums without any allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight time loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  out is a DEVICE
+ * array of 2 doubles (sum |u - v|, sum |v|), written asynchronously on the
+ * stream (no host wait).  The partial sums go to a 16 KiB array of the library
+ * (at most 1024 blocks; every range fits, its rows are strided over them), one
+ * per device for the life of the process; calls that overlap on one device
+ * (other streams, other threads) share it and race. */
+int hsx_l1_sums_noalloc(const hsx_grid *g, const void *u, const void *v, const int64_t range[6], double *out,
+                        void *stream);
 /* max over the range of |u-v|/|v|, or |u-v| where |(float)v| < max_err_float */
 int hsx_max_err(const hsx_grid *g, const void *u, const void *v, const int64_t range[6],
                 double max_err_float, double *out, void *stream);
