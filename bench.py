@@ -100,6 +100,14 @@ def gpu_leg(args):
     capi.fill_separable(g, prn, r_all, *tabs, math.sin(PI_REF * float(np.float32(-2.0 * dt)) * gamma))
     capi.fill_separable(g, prc, r_all, *tabs, math.sin(PI_REF * float(np.float32(-dt)) * gamma))
     capi.sync()
+    tuned = None
+    forced = (args.tile_lx, args.tile_rows, args.tile_chunk, args.streams, args.no_ring) != (0, 0, 0, "default", False)
+    if args.autotune and not forced and args.kernel == "auto":
+        # the tuner times its candidate shapes (the library's default first) on
+        # scratch grids of this layout and keeps the fastest; setup, not timed
+        shape, tms = capi.autotune(g, reps=5)
+        tuned = {"lanes": shape[0], "rows": shape[1], "chunk": shape[2], "flags": shape[3], "ms": round(tms, 4)}
+        log(f"autotuned tile {shape} ({tms:.3f} ms per step in the sweep)")
 
     # a step is the reference's free surface on prc then the fused update of prn;
     # folded (--fold): the first step applies the free surface to the initial
@@ -195,7 +203,7 @@ def gpu_leg(args):
         "kernel_ms": sum(kern_ms) / len(kern_ms),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
-        "npt": npt, "device": info, "dt": float(dt),
+        "npt": npt, "device": info, "dt": float(dt), "tuned": tuned,
         "pointwise_ref": None if pw_ms is None else {
             "kernel": "k_update_pressure16", "bytes_per_point": 20, "ms": pw_ms,
             "GBps": 20 * npt / (pw_ms / 1e3) / 1e9},
@@ -358,7 +366,7 @@ def adjoint_leg(args):
         """best-of-reps seconds of one gradient; with a plan made beforehand
         (the work buffers allocated once per geometry) unless alloc_each, where
         each call allocates and frees them (hsx_gradient)"""
-        plan = None if alloc_each else capi.AdjointPlan(S["g"], nt, len(rec), every, f)
+        plan = None if alloc_each else capi.AdjointPlan(S["g"], nt, len(rec), every, f, autotune=args.autotune)
         best, gout, J = 1e30, None, None
         for _ in range(reps):
             gr = capi.DeviceArray.from_numpy(np.zeros(p.npoint))
@@ -490,6 +498,12 @@ def batch_leg(args):
     b.run(use_graph=True)                   # capture + first replay
     t0 = time.perf_counter()
     b.run(use_graph=True)
+    t_default = time.perf_counter() - t0
+    shape, sms = b.autotune(reps=3)
+    log(f"batch: default shape {t_default:.3f} s; autotuned shape {shape} ({sms:.3f} ms per step in the sweep)")
+    b.run(use_graph=True)                   # capture the tuned step + first replay
+    t0 = time.perf_counter()
+    b.run(use_graph=True)
     t_graph = time.perf_counter() - t0
     t0 = time.perf_counter()
     b.run(use_graph=False)
@@ -509,6 +523,8 @@ def batch_leg(args):
         "gpoint_per_s": round(pts * nt / t_graph / 1e9, 3),
         "roofline_frac": round(16 * pts * nt / t_graph / 1e9 / HBM_PEAK_GBS, 4),
         "s_graph": round(t_graph, 4), "s_no_graph": round(t_nograph, 4), "s_one_by_one": round(t_single, 4),
+        "s_graph_default_shape": round(t_default, 4),
+        "tile": {"lanes": shape[0], "rows": shape[1], "chunk": shape[2], "ring": shape[3]},
         "traces_equal_single_shot": bool(same),
     }
 
@@ -536,6 +552,8 @@ def main():
                     help="free surface folded into the fused step (hsx_ac2_step_fs) instead of its own launches")
     ap.add_argument("--no-ring", action="store_true",
                     help="load each plane's halo one plane ahead instead of with the queue front")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the library's default tile shape (no hsx_autotune before timing)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one full-grid copy kernel first (counter calibration for profiles)")
     ap.add_argument("--no-batch", dest="batch", action="store_false",
@@ -591,7 +609,8 @@ def main():
                    "inner_points": npt, "kernel": args.kernel,
                    "tile_lx": args.tile_lx, "tile_rows": args.tile_rows, "tile_chunk": args.tile_chunk,
                    "streams": args.streams, "halo_ring": not args.no_ring,
-                   "free_surface": "folded into the step" if args.fold else "own launches before the step"},
+                   "free_surface": "folded into the step" if args.fold else "own launches before the step",
+                   "autotuned_tile": gpu["tuned"]},
         "roofline": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in gpu["roofline"].items()},
         "cpu_baseline": cpu,
         "adjoint": adj,

@@ -113,6 +113,7 @@ SIGNATURES = {
     "hsx_autotune_clear": (_i, []),
     "hsx_batch_create": (_i, [_i, _G, _P(_vp), _vp, _P(_vp), _P(_vp)]),
     "hsx_batch_run": (_i, [_vp, _i, _vp]),
+    "hsx_batch_autotune": (_i, [_vp, _i, _P(_i), _P(ctypes.c_float)]),
     "hsx_batch_destroy": (_i, [_vp]),
 }
 
@@ -127,7 +128,7 @@ class HsxAdjointOpts(ctypes.Structure):
     _fields_ = [("ckpt_every", ctypes.c_int), ("ckpt_format", ctypes.c_int), ("flags", ctypes.c_int)]
 
 
-HSX_ADJ_RESIDUAL = 1
+HSX_ADJ_RESIDUAL, HSX_ADJ_AUTOTUNE = 1, 2
 
 
 HSX_INJECT_ADD, HSX_INJECT_SET = 0, 1
@@ -428,6 +429,13 @@ class Batch:
     def run(self, use_graph=True, stream=None):
         check(lib().hsx_batch_run(self.h, 1 if use_graph else 0, stream))
 
+    def autotune(self, reps=3):
+        """hsx_batch_autotune: ((lanes, rows, chunk, ring), ms per step)"""
+        shape = (ctypes.c_int * 4)()
+        ms = ctypes.c_float(0.0)
+        check(lib().hsx_batch_autotune(self.h, int(reps), shape, ctypes.byref(ms)))
+        return tuple(shape), ms.value
+
     def close(self):
         if self.h is not None and self.h.value:
             lib().hsx_batch_destroy(self.h)
@@ -444,10 +452,10 @@ def forward(g: HsxGrid, coef, shot: HsxShot, traces, stream=None):
     check(lib().hsx_forward(ctypes.byref(g), dptr(coef), ctypes.byref(shot), dptr(traces), stream))
 
 
-def _opts(ckpt_every, ckpt_format, residual):
+def _opts(ckpt_every, ckpt_format, residual, autotune=False):
     o = HsxAdjointOpts()
     o.ckpt_every, o.ckpt_format = int(ckpt_every), int(ckpt_format)
-    o.flags = HSX_ADJ_RESIDUAL if residual else 0
+    o.flags = (HSX_ADJ_RESIDUAL if residual else 0) | (HSX_ADJ_AUTOTUNE if autotune else 0)
     return o
 
 
@@ -465,8 +473,9 @@ class AdjointPlan:
     """hsx_adjoint_plan: the gradient's work buffers allocated once for one
     grid layout, nt and receiver count."""
 
-    def __init__(self, g: HsxGrid, nt, nrec, ckpt_every=0, ckpt_format=HSX_CKPT_FP32, residual=False):
-        self._o = _opts(ckpt_every, ckpt_format, residual)
+    def __init__(self, g: HsxGrid, nt, nrec, ckpt_every=0, ckpt_format=HSX_CKPT_FP32, residual=False,
+                 autotune=False):
+        self._o = _opts(ckpt_every, ckpt_format, residual, autotune)
         h = ctypes.c_void_p()
         check(lib().hsx_adjoint_plan_create(ctypes.byref(g), int(nt), int(nrec), ctypes.byref(self._o),
                                             ctypes.byref(h)))

@@ -141,7 +141,8 @@ static int plan_checks(const hsx_grid *g, int64_t nt, int64_t nrec, const hsx_ad
     if (fmt != HSX_CKPT_FP32 && g->storage != HSX_FP32)
         return fail(HSX_ERR_UNSUPPORTED, "16-bit checkpoints of FP64 wavefields");
     if (K == 0 && fmt != HSX_CKPT_FP32) return fail(HSX_ERR_INVALID, "full storage keeps every step in the storage type");
-    if (opt->flags & ~HSX_ADJ_RESIDUAL) return fail(HSX_ERR_INVALID, "adjoint flags 0x%x", opt->flags);
+    if (opt->flags & ~(HSX_ADJ_RESIDUAL | HSX_ADJ_AUTOTUNE))
+        return fail(HSX_ERR_INVALID, "adjoint flags 0x%x", opt->flags);
     for (int a = 0; a < g->dim; a++)
         if (g->inner[2 * a + 1] - g->inner[2 * a] < 2) return fail(HSX_ERR_INVALID, "grid too small for the imaging range");
     return ensure_init();
@@ -311,6 +312,7 @@ int hsx_adjoint_plan_create(const hsx_grid *g, int64_t nt, int64_t nrec, const h
     *out = nullptr;
     int rc = plan_checks(g, nt, nrec, opt);
     if (rc) return rc;
+    if ((opt->flags & HSX_ADJ_AUTOTUNE) && g->dim == 3 && (rc = hsx_autotune(g, 5, nullptr, nullptr))) return rc;
     hsx_adjoint_plan *P = new hsx_adjoint_plan();
     P->g = *g;
     P->nt = nt;

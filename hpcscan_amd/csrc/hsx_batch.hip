@@ -74,6 +74,10 @@ struct hsx_batch {
     int64_t n1_rows = 0;                       // largest number of rows of an n1 face
     void *sr[2] = {nullptr, nullptr};          // SrcRec per shot, per parity
     std::function<int(int, hipStream_t)> step; // batched fused step, per parity
+    std::vector<void *> step_owned;            // device argument tables of step
+    int shape[4] = {0, 0, 0, 0};               // lanes, rows, chunk (0: the rule), ring of step
+    std::vector<hsx_grid> grids;               // the shots' descriptors and coef buffers,
+    std::vector<const void *> coefs;           // to rebuild step (hsx_batch_autotune)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t graph_stream = nullptr;
@@ -97,10 +101,11 @@ static int upload(hsx_batch *b, const void *host, size_t bytes, void **p)
     return HSX_OK;
 }
 
-// the fused-step instance shared by every shot: 64 lanes, 16 rows with the
-// halo ring where it fits, else 16 rows without it, else 8 rows
+// one fused-step instance shared by every shot; force_chunk > 0: planes per
+// workgroup, else whole columns when the tiles fill the device once and
+// 32-plane chunks when they do not
 template <typename TS, int R, int LX, int BY, bool ring>
-static int batch_setup_shape(hsx_batch *b, const hsx_grid *grids, const void *const *coef)
+static int batch_setup_shape(hsx_batch *b, const hsx_grid *grids, const void *const *coef, int64_t force_chunk = 0)
 {
     constexpr int VW = 16 / sizeof(TS);
     using K = Tiled<TS, TS, R, VW, LX, BY, MODE_AC2, ring>;
@@ -118,7 +123,7 @@ static int batch_setup_shape(hsx_batch *b, const hsx_grid *grids, const void *co
     // in several rounds: short chunks then balance them (32 shots of
     // 192^3-320^3, order 8: 2.47 s for 1000 steps with 32-plane chunks, 2.75 s
     // with whole columns; 16-40 planes all within 1 %)
-    int64_t tiles = 0, chunk = 0;
+    int64_t tiles = 0, chunk = force_chunk;
     const TileCfg cfg = tile_forced();
     for (int pass = 0; pass < 2; pass++) {
         tiles = 0;
@@ -134,15 +139,35 @@ static int batch_setup_shape(hsx_batch *b, const hsx_grid *grids, const void *co
             off[s + 1] = off[s] + args[0][s].ntx * args[0][s].nty * args[0][s].nchunk;
             tiles += (int64_t)args[0][s].ntx * args[0][s].nty;
         }
-        if (tiles <= resident) break;
+        if (tiles <= resident || force_chunk > 0) break;
         chunk = 32;
     }
+    // argument tables of this step; those of the step it replaces are freed
+    std::vector<void *> mine;
+    auto up = [&](const void *host, size_t bytes, void **p) -> int {
+        hipError_t e = hipMalloc(p, bytes ? bytes : 1);
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc (batch step)");
+        mine.push_back(*p);
+        HSX_HIP(hipMemcpy(*p, host, bytes, hipMemcpyHostToDevice));
+        return HSX_OK;
+    };
     void *dargs[2], *doff;
     int rc;
-    if ((rc = upload(b, args[0].data(), ns * sizeof(TiledArgs<TS>), &dargs[0])) ||
-        (rc = upload(b, args[1].data(), ns * sizeof(TiledArgs<TS>), &dargs[1])) ||
-        (rc = upload(b, off.data(), (ns + 1) * sizeof(int), &doff)))
+    if ((rc = up(args[0].data(), ns * sizeof(TiledArgs<TS>), &dargs[0])) ||
+        (rc = up(args[1].data(), ns * sizeof(TiledArgs<TS>), &dargs[1])) ||
+        (rc = up(off.data(), (ns + 1) * sizeof(int), &doff))) {
+        for (void *q : mine) (void)hipFree(q);
         return rc;
+    }
+    if (!b->step_owned.empty()) {
+        HSX_HIP(hipDeviceSynchronize());   // no launch of the old step may still read its tables
+        for (void *q : b->step_owned) (void)hipFree(q);
+    }
+    b->step_owned = mine;
+    b->shape[0] = LX;
+    b->shape[1] = BY;
+    b->shape[2] = (int)force_chunk;
+    b->shape[3] = ring ? 1 : 0;
     const int total = off[ns];
     const unsigned grid = (unsigned)(((int64_t)total + 7) / 8 * 8);
     b->step = [=](int parity, hipStream_t s) -> int {
@@ -178,6 +203,33 @@ static int batch_setup_step(hsx_batch *b, const hsx_grid *grids, const void *con
             return batch_setup_shape<TS, R, 32, 16, true>(b, grids, coef);
     }
     return batch_setup_shape<TS, R, LX, 8, false>(b, grids, coef);
+}
+
+// a batch step of the given shape, if it fits this order and precision
+// (LDS, registers without scratch); HSX_ERR_UNSUPPORTED otherwise
+template <typename TS, int R, int LX, int BY, bool RING>
+static int batch_try_shape(hsx_batch *b, int64_t chunk)
+{
+    constexpr int VW = 16 / sizeof(TS);
+    if constexpr (Tiled<TS, TS, R, VW, LX, BY, MODE_AC2, RING>::LDS > TILE_LDS_MAX ||
+                  (RING && !Tiled<TS, TS, R, VW, LX, BY, MODE_AC2, RING>::RING)) {
+        return HSX_ERR_UNSUPPORTED;
+    } else {
+        if (tiled_spills(k_tiled_batch<TS, TS, R, VW, LX, BY, MODE_AC2, RING>)) return HSX_ERR_UNSUPPORTED;
+        return batch_setup_shape<TS, R, LX, BY, RING>(b, b->grids.data(), b->coefs.data(), chunk);
+    }
+}
+
+template <typename TS, int R>
+static int batch_shape_rt(hsx_batch *b, int lx, int by, bool ring, int64_t chunk)
+{
+    if (lx == 64 && by == 16) return ring ? batch_try_shape<TS, R, 64, 16, true>(b, chunk)
+                                          : batch_try_shape<TS, R, 64, 16, false>(b, chunk);
+    if (lx == 32 && by == 16) return ring ? batch_try_shape<TS, R, 32, 16, true>(b, chunk)
+                                          : batch_try_shape<TS, R, 32, 16, false>(b, chunk);
+    if (lx == 64 && by == 8) return batch_try_shape<TS, R, 64, 8, false>(b, chunk);
+    if (lx == 32 && by == 8) return batch_try_shape<TS, R, 32, 8, false>(b, chunk);
+    return HSX_ERR_UNSUPPORTED;
 }
 
 static int batch_record(hsx_batch *b, hipStream_t s)
@@ -240,6 +292,8 @@ int hsx_batch_create(int nshots, const hsx_grid *grids, const void *const *coef,
     }
     hsx_batch *b = new hsx_batch();
     b->nshots = nshots;
+    b->grids.assign(grids, grids + nshots);
+    b->coefs.assign(coef, coef + nshots);
     b->storage = g0.storage;
     b->nt = shots[0].nt;
     const size_t es = g0.storage == HSX_FP64 ? 8 : 4;
@@ -350,6 +404,78 @@ int hsx_batch_run(hsx_batch *b, int use_graph, void *stream)
     return HSX_OK;
 }
 
+int hsx_batch_autotune(hsx_batch *b, int reps, int shape[4], float *best_ms)
+{
+    if (!b) return fail(HSX_ERR_INVALID, "null batch");
+    if (reps < 1 || reps > 1000) return fail(HSX_ERR_INVALID, "reps %d not in 1..1000", reps);
+    int rc = ensure_init();
+    if (rc) return rc;
+    hipStream_t s = stream_of(nullptr);
+    const int R = b->grids[0].fd_order / 2;
+    auto setup = [&](int lx, int by, bool ring, int64_t chunk) -> int {
+        return dispatch_R(R, [&](auto rc_) -> int {
+            constexpr int RR = decltype(rc_)::value;
+            if (b->storage == HSX_FP64) return batch_shape_rt<double, RR>(b, lx, by, ring, chunk);
+            return batch_shape_rt<float, RR>(b, lx, by, ring, chunk);
+        });
+    };
+    struct Cand {
+        int lx, by;
+        bool ring;
+        int chunk;
+    };
+    // the shape hsx_batch_create chose first, then the others
+    std::vector<Cand> cand = {{b->shape[0], b->shape[1], b->shape[3] != 0, b->shape[2]}};
+    for (int lx : {64, 32})
+        for (int by : {16, 8})
+            for (bool ring : {true, false})
+                for (int chunk : {0, 16, 32, 64}) {
+                    if (by == 8 && ring) continue;
+                    const Cand c = {lx, by, ring, chunk};
+                    if (c.lx == cand[0].lx && c.by == cand[0].by && c.ring == cand[0].ring && c.chunk == cand[0].chunk)
+                        continue;
+                    cand.push_back(c);
+                }
+    hipEvent_t e0, e1;
+    HSX_HIP(hipEventCreate(&e0));
+    HSX_HIP(hipEventCreate(&e1));
+    Cand best = cand[0];
+    float best_t = 1e30f;
+    for (const Cand &c : cand) {
+        rc = setup(c.lx, c.by, c.ring, c.chunk);
+        if (rc == HSX_ERR_UNSUPPORTED) continue;
+        if (rc) break;
+        if ((rc = b->step(0, s))) break;                         // warm-up
+        (void)hipEventRecord(e0, s);
+        for (int k = 0; k < reps && !rc; k++) rc = b->step(k & 1, s);
+        (void)hipEventRecord(e1, s);
+        if (rc) break;
+        (void)hipEventSynchronize(e1);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        ms /= reps;
+        if (ms < best_t) {
+            best_t = ms;
+            best = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc == HSX_ERR_UNSUPPORTED) rc = HSX_OK;
+    if (!rc && best_t >= 1e29f) rc = fail(HSX_ERR_UNSUPPORTED, "batch autotune: no shape fits");
+    if (!rc) rc = setup(best.lx, best.by, best.ring, best.chunk);
+    if (rc) return rc;
+    // the captured time loop (if any) launched the old step: capture again
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    if (shape)
+        for (int k = 0; k < 4; k++) shape[k] = b->shape[k];
+    if (best_ms) *best_ms = best_t;
+    return HSX_OK;
+}
+
 int hsx_batch_destroy(hsx_batch *b)
 {
     if (!b) return HSX_OK;
@@ -357,6 +483,7 @@ int hsx_batch_destroy(hsx_batch *b)
     if (b->graph) (void)hipGraphDestroy(b->graph);
     (void)hipDeviceSynchronize();
     for (void *p : b->owned) (void)hipFree(p);
+    for (void *p : b->step_owned) (void)hipFree(p);
     delete b;
     return HSX_OK;
 }

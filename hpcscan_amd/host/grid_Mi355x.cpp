@@ -92,6 +92,15 @@ Rtn_code Grid_Mi355x::initializeGrid(void)
     if (rc != RTN_CODE_OK) return rc;
     if (!check(initLibrary(), "hsx_init")) return RTN_CODE_KO;
     if (!check(hsx_malloc(&d_grid, npoint * sizeof(Myfloat)), "hsx_malloc")) return RTN_CODE_KO;
+    // points no fill reaches (the halos of a coefficient grid) hold zero, not
+    // what the allocation left there: the -writeGrid files are then the same
+    // from run to run
+    {
+        const hsx_grid d = desc();
+        if (!check(hsx_array_op(&d, HSX_ARRAY_FILL, d_grid, nullptr, nullptr, 0.0, nullptr), "hsx_array_op") ||
+            !check(hsx_stream_sync(nullptr), "hsx_stream_sync"))
+            return RTN_CODE_KO;
+    }
     if (mi355xSettings().autotune && dim == DIM3) {
         // once per layout in this process; the library keeps the choice for
         // the fused step and the Laplacian of every grid of that layout

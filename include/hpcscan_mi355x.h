@@ -259,11 +259,14 @@ int hsx_forward(const hsx_grid *g, const void *coef, const hsx_shot *shot, void 
  * loss with respect to the traces): res = d_obs instead of traces - d_obs, so
  * that it reaches the adjoint without being rounded against the traces */
 #define HSX_ADJ_RESIDUAL 1
+/* hsx_adjoint_plan_create tunes the tile shape of the grid's layout
+ * (hsx_autotune) before it allocates the work buffers */
+#define HSX_ADJ_AUTOTUNE 2
 
 typedef struct {
     int ckpt_every;            /* steps per checkpoint segment; 0 = keep every wavefield */
     int ckpt_format;           /* HSX_CKPT_FP32 / FP16 / BF16 (16-bit: FP32 grids only) */
-    int flags;                 /* 0 or HSX_ADJ_RESIDUAL */
+    int flags;                 /* 0 or HSX_ADJ_RESIDUAL | HSX_ADJ_AUTOTUNE */
 } hsx_adjoint_opts;
 
 /* grad (DEVICE, double, npoint values) += dJ/dm for J = 1/2 sum res^2,
@@ -300,6 +303,15 @@ int hsx_batch_create(int nshots, const hsx_grid *grids, const void *const *coef,
 /* all nt steps from zero wavefields; use_graph: capture the time loop into a
  * HIP graph on the first call, replay it on every call */
 int hsx_batch_run(hsx_batch *b, int use_graph, void *stream);
+/* Tile autotuner of a batch (no counterpart in the reference): times the
+ * batched fused step with each candidate shape -- lanes x rows (64 / 32 x 16 / 8),
+ * halo ring on / off, n3 chunk (the rule, 16, 32, 64 planes) -- for reps
+ * launches, starting with the shape hsx_batch_create chose, and keeps the
+ * fastest for later runs (a captured time loop is captured again).
+ * shape[4] (may be NULL) = lanes, rows, chunk (0: the rule), ring of the
+ * choice; *best_ms (may be NULL) its time per step.  The wavefields are
+ * overwritten (every run starts from zero anyway). */
+int hsx_batch_autotune(hsx_batch *b, int reps, int shape[4], float *best_ms);
 int hsx_batch_destroy(hsx_batch *b);
 
 #ifdef __cplusplus

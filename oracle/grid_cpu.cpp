@@ -6,6 +6,7 @@
 // (src/grid_CacheBlk.cpp, block sizes -cb1/-cb2/-cb3).  The product binary
 // never links this file; it is the CPU reference that tests and the bench's
 // cpu_baseline leg run beside the MI355X backend.
+#include <algorithm>
 #include <cstdlib>
 #include <vector>
 
@@ -34,6 +35,16 @@ public:
         : Grid(t, d, a, b, c), blocked(mode)
     {
         setMode();
+    }
+
+    // points no fill reaches (e.g. the halos of the coefficient grid, which
+    // the reference leaves uninitialised) start at zero, so that the files of
+    // -writeGrid are the same from run to run and backend to backend
+    Rtn_code initializeGrid(void) override
+    {
+        const Rtn_code rc = Grid::initializeGrid();
+        if (rc == RTN_CODE_OK && grid_3d) std::fill(grid_3d, grid_3d + npoint, (Myfloat)0);
+        return rc;
     }
 
     void info(void) override

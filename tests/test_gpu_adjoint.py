@@ -312,3 +312,48 @@ def test_indices_outside_the_grid_are_refused_on_the_host(capi):
     for bad in ([-1], [p.npoint], [3, p.npoint + 5]):
         with pytest.raises(ValueError):
             capi.check_indices(bad, p.npoint)
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_batch_autotune_keeps_results(oracle, capi, fp64):
+    """hsx_batch_autotune times the candidate shapes of the batched step and
+    keeps the fastest; the batch then still gives the oracle's traces, with
+    and without the captured graph (captured again after the new shape)"""
+    sizes = [(36, 20, 18), (68, 36, 20), (40, 24, 22)]
+    shots = []
+    for k, n in enumerate(sizes):
+        sh = make_shot(oracle, n, 8, nt=24, fp64=fp64, seed=70 + k, with_obs=False)
+        d, s = _shot_dev(capi, sh)
+        tr = capi.DeviceArray(sh["nt"] * len(sh["rec"]) * np.dtype(sh["dtype"]).itemsize)
+        shots.append((sh, d, s, tr))
+    b = capi.Batch([capi.hsx_grid(x[0]["p"]) for x in shots], [x[1]["coef"] for x in shots],
+                   [x[2] for x in shots], [x[3] for x in shots])
+    b.run(use_graph=True)                   # capture with the first shape
+    shape, ms = b.autotune(reps=2)
+    print(f"batch autotune fp64={fp64}: shape {shape}, {ms:.4f} ms per step", flush=True)
+    assert shape[0] in (32, 64) and shape[1] in (8, 16) and ms > 0
+    for use_graph in (True, False):
+        b.run(use_graph=use_graph)
+        for sh, d, s, tr in shots:
+            ref = oracle.forward(sh["o"], sh["coef"], sh["iv"], sh["nt"], sh["src"], sh["wavelet"], sh["rec"])
+            _same(tr.download(sh["dtype"]).reshape(ref.shape), ref)
+    b.close()
+
+
+def test_plan_with_autotune_bit_exact(oracle, capi):
+    """a plan made with HSX_ADJ_AUTOTUNE (the layout's tile shape tuned first)
+    gives the oracle's gradient bit for bit"""
+    sh = make_shot(oracle, (40, 36, 32), 8, nt=30, seed=37)
+    g = capi.hsx_grid(sh["p"])
+    d, s = _shot_dev(capi, sh)
+    capi.autotune_clear()
+    try:
+        plan = capi.AdjointPlan(g, sh["nt"], len(sh["rec"]), 6, capi.HSX_CKPT_FP32, autotune=True)
+        grad = _dev(capi, np.zeros(sh["p"].npoint))
+        plan.gradient(g, d["coef"], s, _dev(capi, sh["d_obs"]), grad)
+        plan.close()
+        gref, _ = oracle.gradient(sh["o"], sh["coef"], sh["iv"], sh["dt"], sh["nt"], sh["src"], sh["wavelet"],
+                                  sh["rec"], sh["d_obs"])
+        _same(grad.download(np.float64), gref)
+    finally:
+        capi.autotune_clear()

@@ -189,3 +189,56 @@ def test_autotune_flag_same_report(tmp_path, prec):
     assert report_value(gpu, "MI355X autotuned tile") is not None, gpu[-3000:]
     print(report_value(gpu, "MI355X autotuned tile"))
     assert _errors(gpu) == _errors(cpu)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("args", [
+    ["-testCase", "Propa", "-dim", "3", "-n1", "40", "-n2", "36", "-n3", "32", "-fdOrder", "8", "-nt", "12",
+     "-snapInc", "4"],
+    ["-testCase", "Propa", "-dim", "2", "-n1", "44", "-n2", "38", "-fdOrder", "4", "-nt", "10",
+     "-propagator", "Ac2SplitComp"],
+    ["-testCase", "FD_D2", "-dim", "3", "-n1", "30", "-n2", "28", "-n3", "26", "-fdOrder", "12"],
+])
+def test_write_grid_files_equal_cpu(tmp_path, prec, args):
+    """-writeGrid (Grid::write, src/grid.cpp:1245-1310): every .grid.info /
+    .grid.bin file the MI355X run writes equals the CPU run's byte for byte"""
+    gdir, cdir = tmp_path / "gpu", tmp_path / "cpu"
+    gdir.mkdir()
+    cdir.mkdir()
+    rc1, gpu = run_bin(binary(prec), args + ["-testMode", "MI355X", "-ntry", "1", "-writeGrid"], gdir)
+    rc2, cpu = run_bin(CPU_BIN[prec], args + ["-testMode", "CacheBlk", "-ntry", "1", "-writeGrid"], cdir)
+    assert rc1 == 0 and rc2 == 0, gpu[-2000:]
+    gfiles = sorted(f for f in os.listdir(gdir) if ".grid." in f)
+    cfiles = sorted(f for f in os.listdir(cdir) if ".grid." in f)
+    print(prec, " ".join(args), gfiles, flush=True)
+    assert gfiles == cfiles and len(gfiles) >= 2
+    assert any(f.endswith(".grid.bin") for f in gfiles) and any(f.endswith(".grid.info") for f in gfiles)
+    for f in gfiles:
+        a, b = open(gdir / f, "rb").read(), open(cdir / f, "rb").read()
+        assert len(a) == len(b) > 0, f
+        assert a == b, f
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_write_global_snapshots_equal_cpu(tmp_path, prec):
+    """-testCase Modeling -writeGrid: the snapshots written by writeGlobal
+    (src/grid.cpp:1358-1441) equal the CPU run's byte for byte"""
+    from test_harness_cpu import MODELING_ARGS, write_vp_model
+
+    n = (48, 40, 36)
+    write_vp_model(tmp_path / "vp.bin", n, fp64=prec == "fp64")
+    args = MODELING_ARGS + ["-dim", "3", "-modelVpFile", str(tmp_path / "vp.bin"), "-n1", "48", "-n2", "40",
+                            "-n3", "36", "-writeGrid"]
+    gdir, cdir = tmp_path / "gpu", tmp_path / "cpu"
+    gdir.mkdir()
+    cdir.mkdir()
+    rc1, gpu = run_bin(binary(prec), args + ["-testMode", "MI355X"], gdir)
+    rc2, cpu = run_bin(CPU_BIN[prec], args + ["-testMode", "CacheBlk"], cdir)
+    assert rc1 == 0 and rc2 == 0, gpu[-2000:]
+    gfiles = sorted(f for f in os.listdir(gdir) if ".grid." in f)
+    cfiles = sorted(f for f in os.listdir(cdir) if ".grid." in f)
+    print(prec, gfiles, flush=True)
+    assert gfiles == cfiles and any(f.endswith(".grid.bin") for f in gfiles)
+    for f in gfiles:
+        a, b = open(gdir / f, "rb").read(), open(cdir / f, "rb").read()
+        assert len(a) == len(b) > 0 and a == b, f
