@@ -81,6 +81,8 @@ SIGNATURES = {
     "hsx_set_kernel": (_i, [_i]),
     "hsx_get_kernel": (_i, []),
     "hsx_set_tile": (_i, [_i, _i, _i, _i]),
+    "hsx_set_contract": (_i, [_i]),
+    "hsx_get_contract": (_i, []),
     "hsx_fill_const": (_i, [_G, _vp, _R, _d, _vp]),
     "hsx_fill_separable": (_i, [_G, _vp, _R, _P(_d), _P(_d), _P(_d), _d, _vp]),
     "hsx_array_op": (_i, [_G, _i, _vp, _vp, _vp, _d, _vp]),
@@ -188,6 +190,11 @@ def set_tile(lanes_n1: int = 0, chunk_planes: int = 0, rows_n2: int = 0, flags: 
     """Tile shape of the tiled kernels; 0 = the library's choice for each.
     flags: HSX_TILE_CACHED or HSX_TILE_NT (streams), | HSX_TILE_NO_RING."""
     check(lib().hsx_set_tile(lanes_n1, rows_n2, chunk_planes, flags))
+
+
+def set_contract(on: bool):
+    """hsx_set_contract: contracted (FMA) tiled fused step and Laplacian, opt-in."""
+    check(lib().hsx_set_contract(1 if on else 0))
 
 
 def autotune(g: "HsxGrid", reps: int = 3):

@@ -9,11 +9,12 @@
 //   d^n_r = q^{n+1}(x_r),   J = 1/2 sum_{n,r} (d^n_r - dobs^n_r)^2,
 // L the FD Laplacian with the antisymmetric halos of B, which is symmetric on
 // the points B does not zero.  The adjoint field is carried as mu = c lambda:
-//   mu^k = 2 mu^{k+1} - mu^{k+2} + c (L mu^{k+1}) + c e^k,  e^k = residual at step k-1,
-// i.e. the forward step itself (free surface + fused kernel) with the residual
-// injected at the receivers, scaled by c there.  The imaging condition is
+//   mu^k = 2 mu^{k+1} - (mu^{k+2} - c e^k) + c (L mu^{k+1}),  e^k = residual at step k-1,
+// i.e. the forward step itself (free surface + fused kernel) with the residual,
+// scaled by c, taken off mu^{k+2} at the receivers first.  The imaging condition is
 //   dJ/dm = -(1/dt^2) sum_k mu^k (q^k - 2 q^{k-1} + q^{k-2} - F^{k-1}),
-// accumulated in double over the points B does not touch.
+// accumulated in double over the points B does not touch, by the adjoint
+// step's own kernel where the tiled kernels take the grid.
 #include <algorithm>
 #include <vector>
 
@@ -236,13 +237,26 @@ static int plan_run(hsx_adjoint_plan *P, const hsx_grid *g, const void *coef, co
     // ---- backward pass: mu^k for k = nt .. 1
     if ((rc = zero(g, P->m1.p, s)) || (rc = zero(g, P->m2.p, s))) return rc;
     void *mu1 = P->m1.p, *mu2 = P->m2.p;   // mu^{k+1}, mu^{k+2}
+    // the imaging condition fused into the adjoint step's kernel where the
+    // tiled kernels take the grid (exact arithmetic only), else its own pass
+    const bool fuse_img = kernel_family() != HSX_KERNEL_REF && !contract_mode();
     auto back_step = [&](int64_t k, const void *qk, const void *qk1, const void *qk2) -> int {
-        int r = hsx_free_surface(g, mu1, 0x3F, s);
-        if (!r) r = hsx_ac2_step(g, mu2, mu1, coef, s);                    // mu2 <- mu^k
-        if (!r && nrec > 0)
-            r = hsx_inject(g, mu2, nrec, sh->rec_idx, P->res.c() + (size_t)(k - 1) * nrec * es, P->crec.p, 1.0,
+        // mu^k = 2 mu^{k+1} - (mu^{k+2} - c e^k) + c L mu^{k+1}: the residual
+        // goes into mu^{k+2} before the step, so that the step's output is
+        // mu^k complete and the imaging condition can be taken from it
+        int r = HSX_OK;
+        if (nrec > 0)
+            r = hsx_inject(g, mu2, nrec, sh->rec_idx, P->res.c() + (size_t)(k - 1) * nrec * es, P->crec.p, -1.0,
                            HSX_INJECT_ADD, s);
-        if (!r) r = hsx_imaging(g, grad, mu2, qk, qk1, qk2, rim, img_scale, -1, 0.0, s);
+        if (!r) r = hsx_free_surface(g, mu1, 0x3F, s);
+        if (!r) {
+            r = fuse_img ? tiled_ac2_step_img(g, mu2, mu1, coef, grad, qk, qk1, qk2, rim, img_scale, s)
+                         : HSX_ERR_UNSUPPORTED;
+            if (r == HSX_ERR_UNSUPPORTED) {
+                r = hsx_ac2_step(g, mu2, mu1, coef, s);                   // mu2 <- mu^k
+                if (!r) r = hsx_imaging(g, grad, mu2, qk, qk1, qk2, rim, img_scale, -1, 0.0, s);
+            }
+        }
         if (!r && sh->nsrc > 0) {
             r = dispatch_types(g, [&](auto ts, auto) -> int {
                 using TS = decltype(ts);

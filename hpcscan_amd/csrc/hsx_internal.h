@@ -32,6 +32,7 @@ int hip_fail(hipError_t e, const char *what);         // HSX_ERR_HIP with the HI
 int ensure_init(void);                 // lazily select device 0 and create the library stream
 hipStream_t stream_of(void *s);        // NULL -> library stream
 int kernel_family(void);               // HSX_KERNEL_*
+int contract_mode(void);               // hsx_set_contract: 1 = contracted tiled step / Laplacian
 
 // device scratch owned by the library (grown on demand, freed at exit)
 int scratch_get(int slot, size_t bytes, void **ptr);
@@ -162,6 +163,15 @@ int tiled_ac2_step(const hsx_grid *g, void *out, const void *prn, const void *pr
 int tiled_ac2_step_fold(const hsx_grid *g, void *prn, const void *prc, const void *coef, int flags,
                         const int64_t fb[6], hipStream_t s);
 int tiled_laplacian(const hsx_grid *g, const void *u, void *w, hipStream_t s);
+// the contracted (FMA) instances, in translation units of their own
+int tiled_ac2_step_fma(const hsx_grid *g, void *out, const void *prn, const void *prc, const void *coef,
+                       hipStream_t s);
+int tiled_laplacian_fma(const hsx_grid *g, const void *u, void *w, hipStream_t s);
+// the fused step in place on prn with grad += scale * new prn * (q0 - 2 q1 + q2)
+// over box (inside the inner points), in double (the adjoint's imaging condition)
+int tiled_ac2_step_img(const hsx_grid *g, void *prn, const void *prc, const void *coef, double *grad,
+                       const void *q0, const void *q1, const void *q2, const int64_t box[6], double scale,
+                       hipStream_t s);
 int tiled_fd_d2_x(const hsx_grid *g, const void *u, void *w, hipStream_t s);
 int tiled_fd_d2_y(const hsx_grid *g, const void *u, void *w, hipStream_t s);
 int tiled_fd_d2_z(const hsx_grid *g, const void *u, void *w, hipStream_t s);

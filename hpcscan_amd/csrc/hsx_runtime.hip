@@ -32,6 +32,7 @@ struct State {
     int device = 0;
     hipStream_t stream = nullptr;
     int family = HSX_KERNEL_AUTO;
+    int contract = 0;             // hsx_set_contract
     int cus = 0;                  // compute units of the device
     void *scratch[SCRATCH_SLOTS] = {nullptr};
     size_t scratch_bytes[SCRATCH_SLOTS] = {0};
@@ -78,6 +79,8 @@ int ensure_init(void)
 hipStream_t stream_of(void *s) { return s ? (hipStream_t)s : g.stream; }
 
 int kernel_family(void) { return g.family; }
+
+int contract_mode(void) { return g.contract; }
 
 int scratch_get(int slot, size_t bytes, void **ptr)
 {
@@ -339,6 +342,15 @@ int hsx_set_kernel(int family)
 }
 
 int hsx_get_kernel(void) { return g.family; }
+
+int hsx_set_contract(int on)
+{
+    if (on != 0 && on != 1) return fail(HSX_ERR_INVALID, "contract %d not 0 or 1", on);
+    g.contract = on;
+    return HSX_OK;
+}
+
+int hsx_get_contract(void) { return g.contract; }
 
 int hsx_set_tile(int lanes_n1, int rows_n2, int chunk_planes, int flags)
 {

@@ -3,7 +3,8 @@
 // The operation order is the reference's (src/fdm.h:281-545,
 // src/grid.cpp:4476-5112); the library is built with -ffp-contract=off so
 // no a*b+c below becomes an fma and every kernel family returns the same
-// bits as the reference CPU path.
+// bits as the reference CPU path.  The contracted variants (FMA = true,
+// leapfrog_fma) are the opt-in exception: explicit fused multiply-adds.
 #ifndef HSX_STENCIL_POINT_H
 #define HSX_STENCIL_POINT_H
 
@@ -30,8 +31,19 @@ __device__ __forceinline__ TC d2_point(const TS *__restrict__ u, int64_t c, int6
     return acc * inv2;
 }
 
-// same arithmetic from values already gathered: v0 centre, vp[j-1]/vm[j-1] at +-j
-template <typename TS, typename TC, int R>
+// a * b + c rounded once (the opt-in contracted kernels only)
+template <typename TC>
+__device__ __forceinline__ TC fma_tc(TC a, TC b, TC c)
+{
+    if constexpr (sizeof(TC) == 4)
+        return __builtin_fmaf(a, b, c);
+    else
+        return __builtin_fma(a, b, c);
+}
+
+// same arithmetic from values already gathered: v0 centre, vp[j-1]/vm[j-1] at +-j;
+// FMA: each A_j * pair + acc in one fused multiply-add (not the reference's bits)
+template <typename TS, typename TC, int R, bool FMA = false>
 __device__ __forceinline__ TC d2_values(TS v0, const TS *vp, const TS *vm, const TC *A, TC inv2)
 {
     TC acc = A[0] * (TC)v0;
@@ -42,7 +54,7 @@ __device__ __forceinline__ TC d2_values(TS v0, const TS *vp, const TS *vm, const
 #pragma unroll
         for (int j = 1; j <= R; j++) {
             const TS pair = vp[j - 1] + vm[j - 1];
-            acc = acc + A[j] * (TC)pair;
+            acc = FMA ? fma_tc(A[j], (TC)pair, acc) : acc + A[j] * (TC)pair;
         }
     }
     return acc * inv2;
@@ -94,6 +106,14 @@ __device__ __forceinline__ TS leapfrog(TS prc, TS prn, TS coef, TC lap)
 {
     const TC two = (TC)2.0;
     return (TS)(two * (TC)prc - (TC)prn + (TC)coef * lap);
+}
+
+// the same with coef * lap + (2 prc - prn) in one fused multiply-add
+template <typename TS, typename TC>
+__device__ __forceinline__ TS leapfrog_fma(TS prc, TS prn, TS coef, TC lap)
+{
+    const TC two = (TC)2.0;
+    return (TS)fma_tc((TC)coef, lap, two * (TC)prc - (TC)prn);
 }
 
 } // namespace hsx

@@ -30,8 +30,21 @@ namespace hsx {
 
 enum TiledMode {
     MODE_AC2 = 0, MODE_LAP = 1, MODE_D2X = 2, MODE_D2Y = 3, MODE_D2Z = 4,
-    MODE_D1X = 5, MODE_D1Y = 6, MODE_D1Z = 7        // staggered first derivatives
+    MODE_D1X = 5, MODE_D1Y = 6, MODE_D1Z = 7,       // staggered first derivatives
+    // the fused step and the Laplacian with each weighted sum contracted into
+    // fused multiply-adds (opt-in, hsx_set_contract): not the reference's bits
+    MODE_AC2_FMA = 8, MODE_LAP_FMA = 9,
+    // the fused step with the adjoint's imaging condition accumulated from its
+    // output (hsx_gradient): grad += scale * out * (q0 - 2 q1 + q2) in double
+    MODE_AC2_IMG = 10
 };
+constexpr bool mode_ac2(int m) { return m == MODE_AC2 || m == MODE_AC2_FMA || m == MODE_AC2_IMG; }
+constexpr bool mode_lap(int m) { return m == MODE_LAP || m == MODE_LAP_FMA; }
+constexpr bool mode_fma(int m) { return m == MODE_AC2_FMA || m == MODE_LAP_FMA; }
+constexpr int mode_base(int m)
+{
+    return m == MODE_AC2_FMA || m == MODE_AC2_IMG ? MODE_AC2 : m == MODE_LAP_FMA ? MODE_LAP : m;
+}
 
 template <typename T, int VW>
 struct alignas(sizeof(T) * VW) Pack {
@@ -55,6 +68,11 @@ struct TiledArgs {
     //  FOLD_HALO the output's halos of the folded faces are written (-mirror).
     int fold;
     int fb[6];                // boundary plane of each folded face (axis-major, low side first), -1 if not folded
+    // MODE_AC2_IMG: grad[i] += img_scale * out[i] * ((q0[i] - 2 q1[i]) + q2[i]) over img_box
+    const void *img_q[3];
+    double *img_grad;
+    double img_scale;
+    int64_t img_box[6];
     Coefs<TC> c;
 };
 
@@ -72,9 +90,9 @@ struct Tiled {
     static constexpr int PLANE = ROWS * W;            // elements of one LDS plane
     static constexpr int NHG = HL / VW;               // halo groups per side
     static constexpr int NHR = ceil_div(2 * R, BY);   // n2 halo rows loaded per thread
-    static constexpr bool NEED_X = MODE == MODE_AC2 || MODE == MODE_LAP || MODE == MODE_D2X || MODE == MODE_D1X;
-    static constexpr bool NEED_Y = MODE == MODE_AC2 || MODE == MODE_LAP || MODE == MODE_D2Y || MODE == MODE_D1Y;
-    static constexpr bool NEED_Z = MODE == MODE_AC2 || MODE == MODE_LAP || MODE == MODE_D2Z || MODE == MODE_D1Z;
+    static constexpr bool NEED_X = mode_ac2(MODE) || mode_lap(MODE) || MODE == MODE_D2X || MODE == MODE_D1X;
+    static constexpr bool NEED_Y = mode_ac2(MODE) || mode_lap(MODE) || MODE == MODE_D2Y || MODE == MODE_D1Y;
+    static constexpr bool NEED_Z = mode_ac2(MODE) || mode_lap(MODE) || MODE == MODE_D2Z || MODE == MODE_D1Z;
     static constexpr bool D1 = MODE == MODE_D1X || MODE == MODE_D1Y || MODE == MODE_D1Z;
     static constexpr bool NEED_LDS = NEED_X || NEED_Y;
     using V = Pack<TS, VW>;
@@ -195,7 +213,8 @@ __device__ __forceinline__ void tiled_body(const TiledArgs<TC> &a, int logical)
     // Grid::applyBoundaryCondition ANTI_MIRROR, src/grid.cpp:3065-3230): an inner
     // point on a boundary plane of a folded face reads as +0; halos are read as
     // stored (the previous folded step wrote them)
-    constexpr bool FK = FOLD && MODE == MODE_AC2;
+    constexpr bool FK = FOLD && mode_ac2(MODE);
+    constexpr bool FMA = mode_fma(MODE);
     const int fold = FK ? a.fold : 0;
     const int64_t tw = (int64_t)LX * VW;
     auto in_range = [](int64_t x, int64_t lo, int64_t hi) { return x >= lo && x < hi; };
@@ -323,10 +342,10 @@ __device__ __forceinline__ void tiled_body(const TiledArgs<TC> &a, int logical)
     };
     // FP32 fused step of orders 6-10: prn and coef are loaded two planes ahead
     // (registers to spare at 1024 threads; more bytes in flight per CU)
-    constexpr bool DEEP = MODE == MODE_AC2 && sizeof(TS) == 4 && sizeof(TC) == 4 && R >= 3 && R <= 5;
+    constexpr bool DEEP = mode_ac2(MODE) && sizeof(TS) == 4 && sizeof(TC) == 4 && R >= 3 && R <= 5;
     V nprn2 = vzero<TS, VW>(), ncoef2 = vzero<TS, VW>();   // plane z+2 (DEEP)
     auto load_streams_to = [&](int64_t z, V &vp, V &vc) {
-        if (MODE == MODE_AC2 && col_ok) {
+        if (mode_ac2(MODE) && col_ok) {
             if (a.nt) {
                 vp = vload_nt<TS, VW>(at(pb, z, col));
                 vc = vload_nt<TS, VW>(at(cb, z, col));
@@ -531,7 +550,7 @@ __device__ __forceinline__ void tiled_body(const TiledArgs<TC> &a, int logical)
                                 ay[v] = ay[v] + (TC)td.v[v];
                             } else {
                                 const TS pair = tu.v[v] + td.v[v];
-                                ay[v] = ay[v] + A[j] * (TC)pair;
+                                ay[v] = FMA ? fma_tc(A[j], (TC)pair, ay[v]) : ay[v] + A[j] * (TC)pair;
                             }
                         }
                     }
@@ -568,7 +587,7 @@ __device__ __forceinline__ void tiled_body(const TiledArgs<TC> &a, int logical)
                                 p[j - 1] = xr[HL + v + j];
                                 m[j - 1] = xr[HL + v - j];
                             }
-                            d1 = d2_values<TS, TC, R>(c0, p, m, A, a.c.inv2[0]);
+                            d1 = d2_values<TS, TC, R, FMA>(c0, p, m, A, a.c.inv2[0]);
                         }
                         if (K::NEED_Y) d2 = ay[v] * a.c.inv2[1];
                         if (K::NEED_Z) {
@@ -577,14 +596,15 @@ __device__ __forceinline__ void tiled_body(const TiledArgs<TC> &a, int logical)
                                 p[j - 1] = q[(s + R + j) % P].v[v];
                                 m[j - 1] = q[(s + R - j + P) % P].v[v];
                             }
-                            d3 = d2_values<TS, TC, R>(c0, p, m, A, a.c.inv2[2]);
+                            d3 = d2_values<TS, TC, R, FMA>(c0, p, m, A, a.c.inv2[2]);
                         }
                     }
                     TS o;
-                    if (MODE == MODE_AC2) {
+                    if (mode_ac2(MODE)) {
                         const TC lap = d1 + d2 + d3;
-                        o = leapfrog<TS, TC>(c0, prn_v.v[v], coef_v.v[v], lap);
-                    } else if (MODE == MODE_LAP) {
+                        o = FMA ? leapfrog_fma<TS, TC>(c0, prn_v.v[v], coef_v.v[v], lap)
+                                : leapfrog<TS, TC>(c0, prn_v.v[v], coef_v.v[v], lap);
+                    } else if (mode_lap(MODE)) {
                         o = (TS)(d1 + d2 + d3);
                     } else if (MODE == MODE_D2X || MODE == MODE_D1X) {
                         o = (TS)d1;
@@ -608,6 +628,24 @@ __device__ __forceinline__ void tiled_body(const TiledArgs<TC> &a, int logical)
                 }
                 if constexpr (FK) {
                     if (fold & FOLD_HALO) mirror_out(res, z);
+                }
+                if constexpr (MODE == MODE_AC2_IMG) {
+                    // the imaging condition from the values just computed, in
+                    // the order of k_imaging (the box lies inside the inner points)
+                    const int64_t *ib = a.img_box;
+                    if (z >= ib[4] && z <= ib[5] && i2 >= ib[2] && i2 <= ib[3] && i1 + VW - 1 >= ib[0] &&
+                        i1 <= ib[1]) {
+                        const V q0 = vload<TS, VW>(at(reinterpret_cast<const char *>(a.img_q[0]), z, col));
+                        const V q1 = vload<TS, VW>(at(reinterpret_cast<const char *>(a.img_q[1]), z, col));
+                        const V q2 = vload<TS, VW>(at(reinterpret_cast<const char *>(a.img_q[2]), z, col));
+                        double *gp = a.img_grad + (z * a.n12 + i2 * n1 + i1);
+#pragma unroll
+                        for (int v = 0; v < VW; v++) {
+                            if (i1 + v < ib[0] || i1 + v > ib[1]) continue;
+                            const double d2 = ((double)q0.v[v] - 2.0 * (double)q1.v[v]) + (double)q2.v[v];
+                            gp[v] += a.img_scale * (double)res.v[v] * d2;
+                        }
+                    }
                 }
             }
             buf ^= 1;
@@ -664,6 +702,15 @@ struct FoldReq {
     int64_t fb[6];
 };
 extern thread_local FoldReq t_fold;   // defined in hsx_tiled_ac2.hip
+
+// imaging condition fused into the next MODE_AC2_IMG launch of this thread
+struct ImgReq {
+    const void *q[3];
+    double *grad;
+    double scale;
+    int64_t box[6];
+};
+extern thread_local ImgReq t_img;     // defined in hsx_tiled_ac2_img.hip
 constexpr int TILE_LDS_MAX = 163840;  // gfx950: 160 KiB of LDS per workgroup
 
 // 64 lanes unless the tiles of 64 lanes span more than 5/4 of the n1 width
@@ -785,8 +832,19 @@ inline int fill_tiled_args(const hsx_grid *g, void *out, const void *prn_in, con
     a.nchunk = (int)((N3 + len - 1) / len);
     const int streams = cfg.flags & (HSX_TILE_CACHED | HSX_TILE_NT);
     a.nt = streams == HSX_TILE_NT || (streams == 0 && (BY == 16 || RING));
-    a.fold = MODE == MODE_AC2 ? t_fold.flags : 0;
+    a.fold = MODE == MODE_AC2 ? t_fold.flags : 0;   // the folded free surface: exact step only
     for (int k = 0; k < 6; k++) a.fb[k] = a.fold ? (int)t_fold.fb[k] : -1;
+    if constexpr (MODE == MODE_AC2_IMG) {
+        for (int k = 0; k < 3; k++) a.img_q[k] = t_img.q[k];
+        a.img_grad = t_img.grad;
+        a.img_scale = t_img.scale;
+        for (int k = 0; k < 6; k++) a.img_box[k] = t_img.box[k];
+    } else {
+        for (int k = 0; k < 3; k++) a.img_q[k] = nullptr;
+        a.img_grad = nullptr;
+        a.img_scale = 0.0;
+        for (int k = 0; k < 6; k++) a.img_box[k] = 0;
+    }
     a.c = make_coefs<TC>(g);
     if (tiles * a.nchunk > (int64_t)1 << 30) return fail(HSX_ERR_UNSUPPORTED, "too many tiles");
     return HSX_OK;
@@ -926,7 +984,7 @@ inline int run_tiled(const hsx_grid *g, void *out, const void *prn_in, const voi
     const TileCfg forced = tile_forced();
     TileCfg tuned;
     const bool use_tuned = forced.lx == 0 && forced.by == 0 && forced.chunk == 0 && forced.flags == 0 &&
-                           tuned_shape(g, MODE, &tuned);
+                           tuned_shape(g, mode_base(MODE), &tuned);
     return run_tiled_shape<MODE, FOLD>(g, out, prn_in, u, coef, s, use_tuned ? tuned : forced);
 }
 

@@ -37,6 +37,7 @@ thread_local FoldReq t_fold = {0, {-1, -1, -1, -1, -1, -1}};
 
 int tiled_ac2_step(const hsx_grid *g, void *out, const void *prn, const void *prc, const void *coef, hipStream_t s)
 {
+    if (contract_mode()) return tiled_ac2_step_fma(g, out, prn, prc, coef, s);
     return run_tiled<MODE_AC2>(g, out, prn, prc, coef, s);
 }
 

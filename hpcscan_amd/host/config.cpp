@@ -148,6 +148,7 @@ std::vector<Flag> flagTable(Config &c)
         {"-hipFoldBC", K::EnvOn, (void *)"HPCSCAN_MI355X_FOLD_BC", " HIP fold BC\t", C::None, "MI355X: fold the free surface into the fused step"},
         {"-hipNoFoldBC", K::EnvOff, (void *)"HPCSCAN_MI355X_FOLD_BC", " HIP fold BC\t", C::None, "MI355X: apply the free surface as a separate pass (DEFAULT)"},
         {"-hipAutotune", K::EnvOn, (void *)"HPCSCAN_MI355X_AUTOTUNE", " HIP autotune\t", C::None, "MI355X: time the tile shapes once per grid layout, keep the fastest"},
+        {"-hipContract", K::EnvOn, (void *)"HPCSCAN_MI355X_CONTRACT", " HIP contract\t", C::None, "MI355X: fused multiply-adds in the fused step and Laplacian (not bit-exact)"},
     };
 }
 

@@ -36,6 +36,7 @@ const Mi355xSettings &mi355xSettings(void)
         if (const char *v = env("HPCSCAN_MI355X_KERNEL")) x.kernel = v;
         if (const char *v = env("HPCSCAN_MI355X_FOLD_BC")) x.foldBC = atoi(v) != 0;
         if (const char *v = env("HPCSCAN_MI355X_AUTOTUNE")) x.autotune = atoi(v) != 0;
+        if (const char *v = env("HPCSCAN_MI355X_CONTRACT")) x.contract = atoi(v) != 0;
         return x;
     }();
     return s;
@@ -57,6 +58,7 @@ static int initLibrary(void)
         return HSX_ERR_INVALID;
     }
     rc = hsx_set_kernel(fam);
+    if (rc == HSX_OK) rc = hsx_set_contract(c.contract ? 1 : 0);
     if (rc == HSX_OK) g_libReady = true;
     return rc;
 }

@@ -20,12 +20,17 @@ namespace hpcscan {
 //   HPCSCAN_MI355X_FOLD_BC   1: fold the free surface into the fused step (default 0)
 //   HPCSCAN_MI355X_AUTOTUNE  1: time the tile shapes once per 3D grid layout and
 //                            keep the fastest (hsx_autotune; default 0)
-// The harness's -gpuDevice / -hipKernel / -hipFoldBC / -hipAutotune flags set them.
+//   HPCSCAN_MI355X_CONTRACT  1: fused multiply-adds in the tiled fused step and
+//                            Laplacian (hsx_set_contract; results no longer the
+//                            reference's bits; default 0)
+// The harness's -gpuDevice / -hipKernel / -hipFoldBC / -hipAutotune /
+// -hipContract flags set them.
 struct Mi355xSettings {
     int device = 0;
     string kernel = "auto";
     bool foldBC = false;
     bool autotune = false;
+    bool contract = false;
 };
 const Mi355xSettings &mi355xSettings(void);
 

@@ -95,6 +95,17 @@ int         hsx_event_record(void *event, void *stream);
 int         hsx_event_elapsed_ms(void *start, void *stop, float *ms);
 int         hsx_set_kernel(int family);
 int         hsx_get_kernel(void);
+/* Contracted arithmetic (opt-in; no counterpart in the reference): with on = 1
+ * the tiled fused step (hsx_ac2_step, hsx_ac2_step_to) and the tiled Laplacian
+ * (hsx_laplacian) evaluate each weighted sum A_j * (u+j + u-j) + acc, and
+ * coef * lap + (2 prc - prn), as one fused multiply-add: about a third fewer
+ * VALU operations at order 16, where those kernels are bound by VALU issue.
+ * Results then differ from the reference CPU path by rounding (one rounding
+ * per fused pair instead of two), no longer bit for bit.  The reference-order
+ * kernels (grids the tiled kernels do not take) and the folded free surface
+ * stay exact.  Process-wide; default 0. */
+int         hsx_set_contract(int on);
+int         hsx_get_contract(void);
 /* Tile shape of the tiled kernels (0 = the library's choice for each):
  *   lanes_n1      16, 32 or 64 lanes of 16 bytes along n1 (default by grid width)
  *   rows_n2       8 or 16 rows along n2 (16 needs lanes_n1 >= 32 and its LDS

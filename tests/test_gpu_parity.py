@@ -574,3 +574,108 @@ def test_autotuned_shape_bit_exact_and_not_slower(oracle, capi, n, order, prec):
         _same(dw.download(dtype), w)
     finally:
         capi.autotune_clear()
+
+
+@pytest.fixture
+def contracted(capi):
+    capi.set_contract(True)
+    yield
+    capi.set_contract(False)
+
+
+def _rel_l2(a, b):
+    a, b = a.astype(np.float64), b.astype(np.float64)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("order", [4, 8, 12, 16])
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_contracted_kernels_close_to_oracle(oracle, capi, contracted, order, prec):
+    """hsx_set_contract(1): the tiled fused step and Laplacian with fused
+    multiply-adds, one application on random fields, against the exact oracle:
+    within rounding (relative L2 below 1e-6 in FP32, 1e-14 in FP64) and no
+    longer equal (the contraction is active); FD_D2 along one axis stays exact"""
+    dtype, c64 = PRECS[prec]
+    vw = 16 // np.dtype(dtype).itemsize
+    n = (_n1_aligned(72, order, dtype, vw), 40, 36)
+    p, o = _layouts(oracle, 3, n, order, dtype)
+    g = capi.hsx_grid(p, compute_fp64=c64)
+    rng = np.random.default_rng(order * 3 + (1 if c64 else 0))
+    u, coef, prn0 = (_rand(rng, p.npoint, dtype) for _ in range(3))
+    iv = oracle.inv2(p.d, dtype, compute64=c64)
+    du, dc, dp = (capi.DeviceArray.from_numpy(x) for x in (u, coef, prn0))
+    inner = np.zeros((p.n3, p.n2, p.n1), bool)
+    r = p.inner()
+    inner[r[4]:r[5] + 1, r[2]:r[3] + 1, r[0]:r[1] + 1] = True
+    inner = inner.ravel()
+    tol = 1e-6 if dtype == np.float32 else 1e-14
+    ref = prn0.copy()
+    oracle.pressure_fd(o, ref, u, coef, iv, compute64=c64)
+    capi.ac2_step(g, dp, du, dc)
+    got = dp.download(dtype)
+    e1 = _rel_l2(got[inner], ref[inner])
+    w = np.zeros(p.npoint, dtype)
+    dw = capi.DeviceArray.from_numpy(w)
+    oracle.laplacian(o, u, w, iv, compute64=c64)
+    capi.laplacian(g, du, dw)
+    lap = dw.download(dtype)
+    e2 = _rel_l2(lap[inner], w[inner])
+    print(f"contracted O{order} {prec}: fused step rel L2 {e1:.2e}, Laplacian {e2:.2e}", flush=True)
+    assert 0 < e1 < tol and 0 < e2 < tol
+    w1 = np.zeros(p.npoint, dtype)
+    oracle.fd_d2(o, 2, u, w1, iv, compute64=c64)
+    dw.upload(np.zeros(p.npoint, dtype))
+    capi.fd_d2(g, 2, du, dw)
+    _same(dw.download(dtype), w1)
+
+
+@pytest.mark.parametrize("order", [12, 16])
+def test_contracted_propagation_within_1e5(oracle, capi, contracted, order):
+    """the eigen-mode propagation (free surface + contracted fused step, FP32,
+    100^3, 100 steps) against the exact oracle's wavefield: relative L2 within
+    the 1e-5 bar of the FP32 comparisons (1.2e-6 measured at order 12), and
+    the errTestCase within 3 % (it measures the small gap to the eigen mode,
+    which the float rounding of the stencil sums moves by about 1 %)"""
+    import math
+
+    from hpcscan_amd.layout import unit_origin
+
+    n, modes, nt = (100, 100, 100), (2, 2, 2), 100
+    p = make_layout(3, n, order)
+    g = capi.hsx_grid(p)
+    prm = oracle.propa_params(list(n), order, nt_cfg=nt)
+    dt = float(prm["dt"])
+    orig = [unit_origin(p, a) for a in range(3)]
+    gamma = math.sqrt(sum(m * m for m in modes))
+    r_all, r_in = p.all_points(), p.inner()
+    pi = oracle.PI_REF
+    tabs = [np.array([math.sin((orig[a] + i * p.d[a]) * (pi * modes[a]))
+                      for i in range(r_all[2 * a], r_all[2 * a + 1] + 1)]) for a in range(3)]
+
+    def amp(t):
+        return math.sin(pi * float(np.float32(t)) * gamma)
+
+    prn, prc, coef, ref = (capi.DeviceArray(p.npoint * 4) for _ in range(4))
+    for x in (prn, prc, coef, ref):
+        capi.fill_const(g, x, [0, p.n1 - 1, 0, p.n2 - 1, 0, p.n3 - 1], 0.0)
+    capi.fill_const(g, coef, r_in, float(np.float32(np.float32(dt) * np.float32(dt))))
+    capi.fill_separable(g, prn, r_all, *tabs, amp(-2.0 * dt))
+    capi.fill_separable(g, prc, r_all, *tabs, amp(-dt))
+    s1 = s2 = np.float32(0)
+    for it in range(prm["nt"]):
+        capi.free_surface(g, prc)
+        capi.ac2_step(g, prn, prc, coef)
+        capi.fill_separable(g, ref, r_all, *tabs, amp(it * np.float32(dt)))
+        s1 = np.float32(s1 + np.float32(capi.sum_abs_diff(g, prn, ref, r_in)))
+        s2 = np.float32(s2 + np.float32(capi.sum_abs(g, ref, r_in)))
+        prn, prc = prc, prn
+    err_gpu = float(s1 / s2)
+    field = prc.download(np.float32)
+    err_cpu, prn_cpu, _, _ = oracle.propa_eigen(list(n), order, modes, nt_cfg=nt)
+    inner = np.zeros((p.n3, p.n2, p.n1), bool)
+    inner[r_in[4]:r_in[5] + 1, r_in[2]:r_in[3] + 1, r_in[0]:r_in[1] + 1] = True
+    e = _rel_l2(field[inner.ravel()], prn_cpu[inner.ravel()])
+    print(f"contracted O{order} 100^3 x {nt}: wavefield rel L2 {e:.2e}; errTestCase {err_gpu:.6g} vs exact "
+          f"{err_cpu:.6g}", flush=True)
+    assert e < 1e-5
+    assert abs(err_gpu / err_cpu - 1) < 3e-2
