@@ -13,6 +13,10 @@ HBM before timing.  Prints one JSON line.
              (read prc, prn, coef; write prn) / its average HIP-event duration
   cpu_baseline  the oracle's CacheBlk restatement on this host's cores, on a
              time-bounded sample of the same workload
+  analytical_error  errTestCase2 (error against the analytical eigenmode) of
+             the MI355X harness on the CPU baseline's command line beside the
+             CPU path's, and the timed wavefield's own L1 error at its last step
+  batch      32 forward shots of uneven sizes in one batch (HIP graph)
   adjoint    the second half of the metric: seconds per shot of the misfit
              gradient (200^3 order 8, 1000 steps, 16-bit checkpoints) against
              the plain PyTorch full-storage version, with both gradients'
@@ -167,6 +171,17 @@ def gpu_leg(args):
     s = capi.sum_abs(g, b, r_in)
     if not math.isfinite(s) or s <= 0:
         raise RuntimeError(f"wavefield is not finite after the run (sum|p| = {s})")
+    # the timed run's own error against the analytical eigenmode: the newest
+    # field is step it = warmup + steps - 1, time it * dt (in storage precision,
+    # as src/testCase_Propa.cpp:94-106), relative L1 over the inner points
+    it_last = args.warmup + args.steps - 1
+    t_last = float(np.float32(np.float32(it_last) * dt))
+    refg = capi.DeviceArray(nbytes)
+    capi.fill_const(g, refg, full, 0.0)
+    capi.fill_separable(g, refg, r_all, *tabs, math.sin(PI_REF * t_last * gamma))
+    l1_last = capi.sum_abs_diff(g, b, refg, r_in) / capi.sum_abs(g, refg, r_in)
+    refg.free()
+    log(f"analytical error of the timed wavefield (step {it_last}, t = {t_last:.6g} s): relative L1 {l1_last:.6g}")
 
     # pointwise reference on the same grid: updatePressure (prn = 2 prc - prn +
     # coef * lapla, 4 reads + 1 write, no stencil, one thread per 16 bytes in
@@ -205,6 +220,7 @@ def gpu_leg(args):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "npt": npt, "device": info, "dt": float(dt), "tuned": tuned,
+        "l1_last": {"step": it_last, "time_s": t_last, "l1_vs_analytical": l1_last},
         "pointwise_ref": None if pw_ms is None else {
             "kernel": "k_update_pressure16", "bytes_per_point": 20, "ms": pw_ms,
             "GBps": 20 * npt / (pw_ms / 1e3) / 1e9},
@@ -240,20 +256,26 @@ def host_cpu():
 def run_cpu_harness(n, order, nt, ntry, fp64=False, threads=None, timeout=900):
     """oracle/_build/hpcscan_cpu[_fp64] -testCase Propa -testMode CacheBlk with
     the headline command line (eigen mode 2,2,2), OMP_PROC_BIND=close and
-    OMP_PLACES=cores; returns (Best Gpoint/s eff., errTestCase, seconds)."""
-    import subprocess
-
+    OMP_PLACES=cores; returns (Best Gpoint/s eff., errTestCase, seconds, args)."""
     exe = os.path.join(ROOT, "oracle", "_build", "hpcscan_cpu_fp64" if fp64 else "hpcscan_cpu")
     if not os.path.exists(exe):
         raise RuntimeError(f"{exe} is missing: build the oracle (make -C oracle)")
     env = dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores")
     if threads:
         env["OMP_NUM_THREADS"] = str(threads)
-    cmd = [exe, "-testCase", "Propa", "-testMode", "CacheBlk", "-propagator", "Ac2Standard", "-dim", "3",
+    return run_harness(exe, "CacheBlk", n, order, nt, ntry, env, timeout)
+
+
+def run_harness(exe, mode, n, order, nt, ntry, env=None, timeout=900):
+    """exe -testCase Propa -testMode mode with the headline command line (eigen
+    mode 2,2,2); returns (Best Gpoint/s eff., errTestCase, seconds, args)."""
+    import subprocess
+
+    cmd = [exe, "-testCase", "Propa", "-testMode", mode, "-propagator", "Ac2Standard", "-dim", "3",
            "-n1", str(n), "-n2", str(n), "-n3", str(n), "-fdOrder", str(order), "-nt", str(nt),
            "-ntry", str(ntry), "-param1", "2", "-param2", "2", "-param3", "2"]
     t0 = time.perf_counter()
-    r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout, env=env, cwd="/tmp")
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout, env=env or dict(os.environ), cwd="/tmp")
     wall = time.perf_counter() - t0
     if r.returncode != 0:
         raise RuntimeError(f"{' '.join(cmd)} failed ({r.returncode}): {r.stdout[-2000:]}{r.stderr[-2000:]}")
@@ -265,8 +287,27 @@ def run_cpu_harness(n, order, nt, ntry, fp64=False, threads=None, timeout=900):
         elif t.startswith("errTestCase2") or t.startswith("errTestCase"):
             err = float(t.split()[-1])
     if val is None:
-        raise RuntimeError("no 'Best Gpoint/s eff.' line in the CPU harness output")
+        raise RuntimeError(f"no 'Best Gpoint/s eff.' line in the output of {' '.join(cmd)}")
     return val, err, wall, " ".join(cmd[1:])
+
+
+def analytical_leg(args, cpu, gpu):
+    """The reference's analytical-solution error on this workload: the MI355X
+    harness (hpcscan_amd/bin/hpcscan -testMode MI355X) runs the CPU baseline's
+    command line (checks at every snapInc-th step of the first try) and its
+    errTestCase2 line is set beside the CPU path's; plus the timed run's own
+    relative L1 error against the eigenmode at its last step."""
+    exe = os.path.join(ROOT, "hpcscan_amd", "bin", "hpcscan")
+    if not os.path.exists(exe):
+        raise RuntimeError(f"{exe} is missing: build the harness (__graft_entry__.build())")
+    _, err, wall, cmd = run_harness(exe, "MI355X", args.n, args.order, args.cpu_nt, 1, timeout=600)
+    out = {"command": "hpcscan " + cmd, "errTestCase_mi355x": err,
+           "errTestCase_cpu": None if cpu is None else cpu.get("errTestCase"),
+           "timed_wavefield": gpu["l1_last"]}
+    if out["errTestCase_cpu"] is not None and err is not None:
+        out["agree_3_digits"] = f"{err:.3g}" == f"{out['errTestCase_cpu']:.3g}"
+    log(f"analytical error: errTestCase2 MI355X {err} / CPU {out['errTestCase_cpu']} ({wall:.1f} s run)")
+    return out
 
 
 def cpu_leg(args):
@@ -544,6 +585,8 @@ def main():
                     help="time steps of each of the two tries of the bounded CPU-baseline sample "
                          "(about 20 s at 1000^3 on 16 cores)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-analytical", action="store_true",
+                    help="skip the MI355X harness run that reports errTestCase2 on the CPU baseline's command line")
     ap.add_argument("--tile-lx", type=int, default=0, choices=[0, 16, 32, 64],
                     help="lanes along n1 per tile of the fused kernel (0 = chosen from the grid width)")
     ap.add_argument("--tile-rows", type=int, default=0, choices=[0, 8, 16],
@@ -594,6 +637,7 @@ def main():
     adj = None if args.no_adjoint else adjoint_leg(args)
     bat = batch_leg(args) if args.batch else None
     cpu = None if args.no_cpu_baseline else cpu_leg(args)
+    ana = None if args.no_analytical else analytical_leg(args, cpu, gpu)
     npt = gpu["npt"]
     pts_stencil = 6 * (args.order // 2) + 1
     out = {
@@ -620,6 +664,7 @@ def main():
                    "autotuned_tile": gpu["tuned"]},
         "roofline": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in gpu["roofline"].items()},
         "cpu_baseline": cpu,
+        "analytical_error": ana,
         "adjoint": adj,
         "batch": bat,
         "extra": {"fused_kernel_ms": round(gpu["kernel_ms"], 4),

@@ -49,18 +49,74 @@ __global__ void k_sum_partials(const double *partial, int n, double *out)
     }
 }
 
-// grad[src_i] += scale * mu[src_i] * f_i (the source term of the imaging
-// condition), for the sources inside the imaging box b only
+// Sources then receivers of one forward step in one launch (k_inject + k_gather
+// in order): block 0 adds the sources (out += 1.0 * w, in double), waits, then
+// records the receivers that sit on a source point; the other blocks record
+// every other receiver, whose points no source writes.  nsrc <= SRCREC_MAX
+// (the membership test is a loop over the sources).
+constexpr int64_t SRCREC_MAX = 64;
+
 template <typename TS>
-__global__ void k_imaging_src(double *grad, const TS *mu, int64_t nsrc, const int64_t *idx, const TS *f, double scale,
-                              Box b, int64_t n1, int64_t n12)
+__device__ __forceinline__ bool on_source(int64_t j, int64_t nsrc, const int64_t *src)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nsrc) return;
-    const int64_t j = idx[i];
-    const int64_t i3 = j / n12, i2 = (j - i3 * n12) / n1, i1 = j - i3 * n12 - i2 * n1;
-    if (i1 < b.s1 || i1 > b.e1 || i2 < b.s2 || i2 > b.e2 || i3 < b.s3 || i3 > b.e3) return;
-    grad[j] += scale * (double)mu[j] * (double)f[i];
+    for (int64_t k = 0; k < nsrc; k++)
+        if (src[k] == j) return true;
+    return false;
+}
+
+template <typename TS>
+__global__ void __launch_bounds__(256) k_srcrec(TS *u, int64_t npoint, int64_t nsrc, const int64_t *src, const TS *w,
+                                                int64_t nrec, const int64_t *rec, TS *tr)
+{
+    if (blockIdx.x == 0) {
+        for (int64_t i = threadIdx.x; i < nsrc; i += blockDim.x) {
+            if ((uint64_t)src[i] >= (uint64_t)npoint) continue;
+            TS *p = u + src[i];
+            *p = (TS)((double)*p + 1.0 * (double)w[i]);
+        }
+        __syncthreads();
+        for (int64_t i = threadIdx.x; i < nrec; i += blockDim.x) {
+            const int64_t j = rec[i];
+            if (on_source<TS>(j, nsrc, src)) tr[i] = (uint64_t)j < (uint64_t)npoint ? u[j] : (TS)0;
+        }
+        return;
+    }
+    for (int64_t i = (int64_t)(blockIdx.x - 1) * blockDim.x + threadIdx.x; i < nrec;
+         i += (int64_t)(gridDim.x - 1) * blockDim.x) {
+        const int64_t j = rec[i];
+        if (!on_source<TS>(j, nsrc, src)) tr[i] = (uint64_t)j < (uint64_t)npoint ? u[j] : (TS)0;
+    }
+}
+
+// The source term of reverse step k (the imaging condition's -F part:
+// grad[src_i] += scale * mu^k[src_i] * f_i, for the sources inside the imaging
+// box b only) and the residual of step k-1 into
+// mu^{k+1} (v[rec_i] += -1 * res_i * crec_i, in double, as k_inject) in one
+// launch: blocks [0, nb_src) take the sources, the rest the receivers; the two
+// parts touch different buffers.
+template <typename TS>
+__global__ void __launch_bounds__(256) k_imgsrc_residual(double *grad, const TS *mu, int64_t nsrc, const int64_t *sidx,
+                                                         const TS *f, double scale, Box b, int64_t n1, int64_t n12,
+                                                         uint32_t nb_src, TS *v, int64_t npoint, int64_t nrec,
+                                                         const int64_t *ridx, const TS *res, const TS *crec)
+{
+    if (blockIdx.x < nb_src) {
+        const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (i >= nsrc) return;
+        const int64_t j = sidx[i];
+        const int64_t i3 = j / n12, i2 = (j - i3 * n12) / n1, i1 = j - i3 * n12 - i2 * n1;
+        if (i1 < b.s1 || i1 > b.e1 || i2 < b.s2 || i2 > b.e2 || i3 < b.s3 || i3 > b.e3) return;
+        grad[j] += scale * (double)mu[j] * (double)f[i];
+        return;
+    }
+    for (int64_t i = (int64_t)(blockIdx.x - nb_src) * blockDim.x + threadIdx.x; i < nrec;
+         i += (int64_t)(gridDim.x - nb_src) * blockDim.x) {
+        if ((uint64_t)ridx[i] >= (uint64_t)npoint) continue;
+        double x = -1.0 * (double)res[i];
+        x *= (double)crec[i];
+        TS *p = v + ridx[i];
+        *p = (TS)((double)*p + x);
+    }
 }
 
 struct Dev {
@@ -102,6 +158,18 @@ static int fwd_step(const hsx_grid *g, const void *coef, const hsx_shot *sh, voi
 {
     int rc = hsx_free_surface(g, cur, 0x3F, s);
     if (!rc) rc = hsx_ac2_step_to(g, out, prev, cur, coef, s);
+    if (!rc && sh->nsrc > 0 && sh->nsrc <= SRCREC_MAX && sh->nrec > 0 && traces) {
+        const int64_t nrec = sh->nrec;
+        const unsigned nb = 1 + (unsigned)std::min<int64_t>((nrec + 255) / 256, 1024);
+        return dispatch_types(g, [&](auto ts, auto) -> int {
+            using TS = decltype(ts);
+            hipLaunchKernelGGL(k_srcrec<TS>, dim3(nb), dim3(256), 0, s, (TS *)out, g->n1 * g->n2 * g->n3, sh->nsrc,
+                               sh->src_idx, (const TS *)((const char *)sh->wavelet + (size_t)n * sh->nsrc * es), nrec,
+                               sh->rec_idx, (TS *)((char *)traces + (size_t)n * nrec * es));
+            HSX_LAUNCHED("k_srcrec");
+            return HSX_OK;
+        });
+    }
     if (!rc && sh->nsrc > 0)
         rc = hsx_inject(g, out, sh->nsrc, sh->src_idx, (const char *)sh->wavelet + (size_t)n * sh->nsrc * es, nullptr,
                         1.0, HSX_INJECT_ADD, s);
@@ -240,15 +308,17 @@ static int plan_run(hsx_adjoint_plan *P, const hsx_grid *g, const void *coef, co
     // the imaging condition fused into the adjoint step's kernel where the
     // tiled kernels take the grid (exact arithmetic only), else its own pass
     const bool fuse_img = kernel_family() != HSX_KERNEL_REF && !contract_mode();
+    // the residual of step k - 1 goes into mu^{k+1}: for k = nt here, for the
+    // others in the launch that ends reverse step k
+    auto res_at = [&](int64_t k) { return P->res.c() + (size_t)(k - 1) * nrec * es; };
+    if (nrec > 0 &&
+        (rc = hsx_inject(g, mu2, nrec, sh->rec_idx, res_at(nt), P->crec.p, -1.0, HSX_INJECT_ADD, s)))
+        return rc;
     auto back_step = [&](int64_t k, const void *qk, const void *qk1, const void *qk2) -> int {
         // mu^k = 2 mu^{k+1} - (mu^{k+2} - c e^k) + c L mu^{k+1}: the residual
-        // goes into mu^{k+2} before the step, so that the step's output is
+        // went into mu^{k+2} (mu2) before the step, so that the step's output is
         // mu^k complete and the imaging condition can be taken from it
-        int r = HSX_OK;
-        if (nrec > 0)
-            r = hsx_inject(g, mu2, nrec, sh->rec_idx, P->res.c() + (size_t)(k - 1) * nrec * es, P->crec.p, -1.0,
-                           HSX_INJECT_ADD, s);
-        if (!r) r = hsx_free_surface(g, mu1, 0x3F, s);
+        int r = hsx_free_surface(g, mu1, 0x3F, s);
         if (!r) {
             r = fuse_img ? tiled_ac2_step_img(g, mu2, mu1, coef, grad, qk, qk1, qk2, rim, img_scale, s)
                          : HSX_ERR_UNSUPPORTED;
@@ -257,14 +327,21 @@ static int plan_run(hsx_adjoint_plan *P, const hsx_grid *g, const void *coef, co
                 if (!r) r = hsx_imaging(g, grad, mu2, qk, qk1, qk2, rim, img_scale, -1, 0.0, s);
             }
         }
-        if (!r && sh->nsrc > 0) {
+        // the source term of step k (reads mu^k in mu2) and the residual of
+        // step k - 1 into mu^{k+1} (mu1): one launch
+        const int64_t ns = sh->nsrc, nr = k >= 2 ? nrec : 0;
+        if (!r && (ns > 0 || nr > 0)) {
             r = dispatch_types(g, [&](auto ts, auto) -> int {
                 using TS = decltype(ts);
-                hipLaunchKernelGGL(k_imaging_src<TS>, dim3((unsigned)((sh->nsrc + 63) / 64)), dim3(64), 0, s, grad,
-                                   (const TS *)mu2, sh->nsrc, sh->src_idx,
-                                   (const TS *)((const char *)sh->wavelet + (size_t)(k - 1) * sh->nsrc * es),
-                                   -img_scale, rimb, g->n1, g->n1 * g->n2);
-                HSX_LAUNCHED("k_imaging_src");
+                const uint32_t nbs = (uint32_t)((ns + 255) / 256);
+                const uint32_t nbr = (uint32_t)std::min<int64_t>((nr + 255) / 256, 1024);
+                hipLaunchKernelGGL(k_imgsrc_residual<TS>, dim3(nbs + nbr), dim3(256), 0, s, grad, (const TS *)mu2,
+                                   ns, sh->src_idx,
+                                   ns > 0 ? (const TS *)((const char *)sh->wavelet + (size_t)(k - 1) * ns * es)
+                                          : (const TS *)nullptr,
+                                   -img_scale, rimb, g->n1, g->n1 * g->n2, nbs, (TS *)mu1, npoint, nr, sh->rec_idx,
+                                   nr > 0 ? (const TS *)res_at(k - 1) : (const TS *)nullptr, (const TS *)P->crec.p);
+                HSX_LAUNCHED("k_imgsrc_residual");
                 return HSX_OK;
             });
         }

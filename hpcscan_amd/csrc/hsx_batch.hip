@@ -1,6 +1,7 @@
 // Batched shots: many grids (one order and precision, any sizes) advanced
 // together, one launch of each kernel per time step for all of them:
-//   free surface of every shot (2 launches), the fused Ac2 step of every shot
+//   free surface of every shot (2 launches: n1 faces and mirrors, then the
+//   boundary planes), the fused Ac2 step of every shot
 //   (one k_tiled_batch launch over the tiles of all grids), then sources and
 //   receivers of every shot (one launch, one workgroup per shot).
 // The whole time loop can be captured once into a HIP graph and replayed.
@@ -245,13 +246,11 @@ static int batch_record(hsx_batch *b, hipStream_t s)
     HSX_LAUNCHED("k_zero_batch");
     for (int64_t n = 0; n < b->nt; n++) {
         const int p = (int)(n & 1);
-        int rc = HSX_OK;
-        if (b->n1_nh)
-            rc = bc_batch_n1_launch(b->storage, ns, (void *const *)b->bc_fields[p], (const N1Faces *)b->n1, b->n1_nh,
-                                    b->n1_rows, s);
-        if (!rc)
-            rc = bc_batch_launch(b->storage, ns, (void *const *)b->bc_fields[p], (const FaceSet *)b->sets, b->most,
-                                 b->most_plane, s);
+        int rc = b->n1_nh ? bc_batch_fs_launch(b->storage, ns, (void *const *)b->bc_fields[p], (const N1Faces *)b->n1,
+                                               b->n1_nh, b->n1_rows, (const FaceSet *)b->sets, b->most,
+                                               b->most_plane, s)
+                          : bc_batch_launch(b->storage, ns, (void *const *)b->bc_fields[p], (const FaceSet *)b->sets,
+                                            b->most, b->most_plane, s);
         if (!rc) rc = b->step(p, s);
         if (rc) return rc;
         if (b->storage == HSX_FP64)

@@ -152,12 +152,15 @@ __global__ void __launch_bounds__(256) k_free_surface(TS *u, Face f)
 // The face loops index with IT = uint32_t when the face fits (always, below
 // 65536^2 points per face): the 64-bit divisions otherwise dominate these
 // kernels, which move only a few bytes per index.
+// bx / nbx: this block's index and the number of blocks sharing the face (a
+// launch may give other blocks other work).
 template <typename TS, typename IT>
-__device__ __forceinline__ void bc_mirror_loop(TS *u, const FaceSet &fs, const Face &f, IT H, IT NB, IT total)
+__device__ __forceinline__ void bc_mirror_loop(TS *u, const FaceSet &fs, const Face &f, IT H, IT NB, IT total,
+                                               uint32_t bx, uint32_t nbx)
 {
     const int a = f.axis;
     const int oa = a == 2 ? 1 : 2, ob = a == 0 ? 1 : 0;
-    for (IT t = (IT)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (IT)gridDim.x * blockDim.x) {
+    for (IT t = (IT)bx * blockDim.x + threadIdx.x; t < total; t += (IT)nbx * blockDim.x) {
         IT k, ib, io;
         if (a == 0) {
             const IT r = t / H;
@@ -191,7 +194,7 @@ __device__ __forceinline__ void bc_mirror_loop(TS *u, const FaceSet &fs, const F
 // zero: n2 faces 0.020 ms, n3 faces 0.020 ms, against 0.033 / 0.035 ms with one
 // point per thread.
 template <typename TS>
-__device__ __forceinline__ void bc_mirror_vec(TS *u, const FaceSet &fs, const Face &f)
+__device__ __forceinline__ void bc_mirror_vec(TS *u, const FaceSet &fs, const Face &f, uint32_t bx, uint32_t nbx)
 {
     constexpr int VW = 16 / sizeof(TS);
     using NV = TS __attribute__((ext_vector_type(VW)));
@@ -200,7 +203,7 @@ __device__ __forceinline__ void bc_mirror_vec(TS *u, const FaceSet &fs, const Fa
     const uint32_t NG = (uint32_t)(f.be / VW - g0 + 1), H = (uint32_t)(f.he - f.hs + 1);
     const uint32_t total = NG * H * (uint32_t)(f.oe - f.os + 1);
     const bool z1lo = fs.zmask & 1, z1hi = fs.zmask & 2;
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    for (uint32_t t = bx * blockDim.x + threadIdx.x; t < total; t += nbx * blockDim.x) {
         const uint32_t r = t / NG, g = t - r * NG;
         const uint32_t io = r / H, k = r - io * H;
         const int64_t i = f.hs + (int64_t)k;       // halo index along the face axis
@@ -231,20 +234,20 @@ __device__ __forceinline__ void bc_mirror_vec(TS *u, const FaceSet &fs, const Fa
 }
 
 template <typename TS>
-__device__ __forceinline__ void bc_mirror_face(TS *u, const FaceSet &fs, int face)
+__device__ __forceinline__ void bc_mirror_face(TS *u, const FaceSet &fs, int face, uint32_t bx, uint32_t nbx)
 {
     const Face &f = fs.f[face];
     if (fs.vec1 && f.axis != 0) {
-        bc_mirror_vec(u, fs, f);
+        bc_mirror_vec(u, fs, f, bx, nbx);
         return;
     }
     const int64_t H = f.he - f.hs + 1;
     const int64_t NB = f.be - f.bs + 1, NO = f.oe - f.os + 1;
     const int64_t total = H * NB * NO;
-    if (total <= (int64_t)UINT32_MAX - (int64_t)gridDim.x * blockDim.x)
-        bc_mirror_loop<TS, uint32_t>(u, fs, f, (uint32_t)H, (uint32_t)NB, (uint32_t)total);
+    if (total <= (int64_t)UINT32_MAX - (int64_t)nbx * blockDim.x)
+        bc_mirror_loop<TS, uint32_t>(u, fs, f, (uint32_t)H, (uint32_t)NB, (uint32_t)total, bx, nbx);
     else
-        bc_mirror_loop<TS, uint64_t>(u, fs, f, (uint64_t)H, (uint64_t)NB, (uint64_t)total);
+        bc_mirror_loop<TS, uint64_t>(u, fs, f, (uint64_t)H, (uint64_t)NB, (uint64_t)total, bx, nbx);
 }
 
 template <typename TS, typename IT>
@@ -337,12 +340,39 @@ __global__ void __launch_bounds__(256) k_bc_n1(TS *u, Face lo, Face hi, int side
     bc_n1_rows<TS, NH>(u, lo, hi, sides, blockIdx.x * 256u + threadIdx.x);
 }
 
-// many grids: blockIdx.y = grid
+// The n1 faces (bc_n1_rows) and the mirrors of the other faces in one launch:
+// they write disjoint halo points and neither reads what the other writes
+// (bc_n1_rows rewrites the n1 boundary group with its own values and a zero
+// that the other faces' mirrors take as -0 without reading it), so the two
+// need no order; blocks [0, nb_n1) take the n1 rows, the next nb1 * rest.n the
+// mirrors (nb1 blocks per face).  The boundary planes follow in k_bc_zero.
 template <typename TS, int NH>
-__global__ void __launch_bounds__(256) k_bc_n1_batch(TS *const *u, const N1Faces *nf)
+__global__ void __launch_bounds__(256) k_bc_n1_mirror(TS *u, Face lo, Face hi, int sides, FaceSet rest,
+                                                     uint32_t nb_n1, uint32_t nb1)
 {
-    const N1Faces &x = nf[blockIdx.y];
-    if (x.sides) bc_n1_rows<TS, NH>(u[blockIdx.y], x.lo, x.hi, x.sides, blockIdx.x * 256u + threadIdx.x);
+    if (blockIdx.x < nb_n1) {
+        bc_n1_rows<TS, NH>(u, lo, hi, sides, blockIdx.x * 256u + threadIdx.x);
+        return;
+    }
+    const uint32_t m = blockIdx.x - nb_n1;
+    bc_mirror_face(u, rest, (int)(m / nb1), m % nb1, nb1);
+}
+
+// the same for many grids: blockIdx.y = grid; blocks past nb_n1 take the
+// mirrors of faces (blockIdx.x - nb_n1) / nb1 < fs[grid].n
+template <typename TS, int NH>
+__global__ void __launch_bounds__(256) k_bc_n1_mirror_batch(TS *const *u, const N1Faces *nf, const FaceSet *fs,
+                                                           uint32_t nb_n1, uint32_t nb1)
+{
+    const int s = blockIdx.y;
+    if (blockIdx.x < nb_n1) {
+        const N1Faces &x = nf[s];
+        if (x.sides) bc_n1_rows<TS, NH>(u[s], x.lo, x.hi, x.sides, blockIdx.x * 256u + threadIdx.x);
+        return;
+    }
+    const uint32_t m = blockIdx.x - nb_n1;
+    const int face = (int)(m / nb1);
+    if (face < fs[s].n) bc_mirror_face(u[s], fs[s], face, m % nb1, nb1);
 }
 
 int n1_groups_ok(const hsx_grid *g, const void *u, const FaceSet &fs)
@@ -386,7 +416,7 @@ int n1_vector_sides(const hsx_grid *g, const void *u, const FaceSet &fs, Face *l
 template <typename TS>
 __global__ void __launch_bounds__(256) k_bc_mirror(TS *u, FaceSet fs)
 {
-    bc_mirror_face(u, fs, blockIdx.y);
+    bc_mirror_face(u, fs, blockIdx.y, blockIdx.x, gridDim.x);
 }
 
 template <typename TS>
@@ -400,7 +430,7 @@ template <typename TS>
 __global__ void __launch_bounds__(256) k_bc_mirror_batch(TS *const *u, const FaceSet *fs)
 {
     const int s = blockIdx.y / 6, face = blockIdx.y % 6;
-    if (face < fs[s].n) bc_mirror_face(u[s], fs[s], face);
+    if (face < fs[s].n) bc_mirror_face(u[s], fs[s], face, blockIdx.x, gridDim.x);
 }
 
 template <typename TS>
@@ -483,26 +513,31 @@ int bc_batch_launch(int storage, int nshots, void *const *d_fields, const FaceSe
     return HSX_OK;
 }
 
-int bc_batch_n1_launch(int storage, int nshots, void *const *d_fields, const N1Faces *d_n1, int nh, int64_t most_rows,
-                       hipStream_t s)
+int bc_batch_fs_launch(int storage, int nshots, void *const *d_fields, const N1Faces *d_n1, int nh,
+                       int64_t most_rows, const FaceSet *d_sets, int64_t most, int64_t most_plane, hipStream_t s)
 {
-    const dim3 grid((unsigned)std::max<int64_t>(1, (2 * most_rows + 255) / 256), (unsigned)nshots);
-    if (storage == HSX_FP64) {
-        double *const *u = (double *const *)d_fields;
+    if (nh < 1 || nh > 4) return fail(HSX_ERR_INVALID, "batch free surface: %d n1 halo groups", nh);
+    const uint32_t nbn = (uint32_t)std::max<int64_t>(1, (2 * most_rows + 255) / 256);
+    const uint32_t nb1 = (uint32_t)std::max<int64_t>(1, std::min<int64_t>((most + 255) / 256, 4096));
+    const uint32_t nb2 = (uint32_t)std::max<int64_t>(1, std::min<int64_t>((most_plane + 255) / 256, 4096));
+    const dim3 both(nbn + 6 * nb1, (unsigned)nshots);
+    auto launch = [&](auto ts) {
+        using TS = decltype(ts);
+        TS *const *u = (TS *const *)d_fields;
         switch (nh) {
-        case 1: hipLaunchKernelGGL((k_bc_n1_batch<double, 1>), grid, dim3(256), 0, s, u, d_n1); break;
-        case 2: hipLaunchKernelGGL((k_bc_n1_batch<double, 2>), grid, dim3(256), 0, s, u, d_n1); break;
-        case 3: hipLaunchKernelGGL((k_bc_n1_batch<double, 3>), grid, dim3(256), 0, s, u, d_n1); break;
-        default: hipLaunchKernelGGL((k_bc_n1_batch<double, 4>), grid, dim3(256), 0, s, u, d_n1); break;
+        case 1: hipLaunchKernelGGL((k_bc_n1_mirror_batch<TS, 1>), both, dim3(256), 0, s, u, d_n1, d_sets, nbn, nb1); break;
+        case 2: hipLaunchKernelGGL((k_bc_n1_mirror_batch<TS, 2>), both, dim3(256), 0, s, u, d_n1, d_sets, nbn, nb1); break;
+        case 3: hipLaunchKernelGGL((k_bc_n1_mirror_batch<TS, 3>), both, dim3(256), 0, s, u, d_n1, d_sets, nbn, nb1); break;
+        default: hipLaunchKernelGGL((k_bc_n1_mirror_batch<TS, 4>), both, dim3(256), 0, s, u, d_n1, d_sets, nbn, nb1); break;
         }
-    } else {
-        float *const *u = (float *const *)d_fields;
-        switch (nh) {
-        case 1: hipLaunchKernelGGL((k_bc_n1_batch<float, 1>), grid, dim3(256), 0, s, u, d_n1); break;
-        default: hipLaunchKernelGGL((k_bc_n1_batch<float, 2>), grid, dim3(256), 0, s, u, d_n1); break;
-        }
-    }
-    HSX_LAUNCHED("k_bc_n1_batch");
+        HSX_LAUNCHED("k_bc_n1_mirror_batch");
+        hipLaunchKernelGGL(k_bc_zero_batch<TS>, dim3(nb2, 6 * nshots), dim3(256), 0, s, u, d_sets);
+        HSX_LAUNCHED("k_bc_zero_batch");
+    };
+    if (storage == HSX_FP64)
+        launch(0.0);
+    else
+        launch(0.0f);
     return HSX_OK;
 }
 
@@ -834,17 +869,31 @@ int hsx_free_surface(const hsx_grid *g, void *u, int face_mask, void *stream)
                 const Face &f = (sides & 1) ? lo : hi;
                 const int64_t threads = (f.be - f.bs + 1) * (f.oe - f.os + 1) * (sides == 3 ? 2 : 1);
                 const dim3 grid((unsigned)((threads + 255) / 256));
-                switch ((f.he - f.hs + 1) / (16 / (int64_t)sizeof(TS))) {
-                case 1: hipLaunchKernelGGL((k_bc_n1<TS, 1>), grid, dim3(256), 0, s, (TS *)u, lo, hi, sides); break;
-                case 2: hipLaunchKernelGGL((k_bc_n1<TS, 2>), grid, dim3(256), 0, s, (TS *)u, lo, hi, sides); break;
-                case 3: hipLaunchKernelGGL((k_bc_n1<TS, 3>), grid, dim3(256), 0, s, (TS *)u, lo, hi, sides); break;
-                default: hipLaunchKernelGGL((k_bc_n1<TS, 4>), grid, dim3(256), 0, s, (TS *)u, lo, hi, sides); break;
-                }
-                HSX_LAUNCHED("k_bc_n1");
                 rest.n = 0;
                 for (int k = 0; k < fs.n; k++)
                     if (fs.f[k].axis != 0) rest.f[rest.n++] = fs.f[k];
-                if (rest.n == 0) return HSX_OK;
+                if (rest.n == 0) {
+                    switch ((f.he - f.hs + 1) / (16 / (int64_t)sizeof(TS))) {
+                    case 1: hipLaunchKernelGGL((k_bc_n1<TS, 1>), grid, dim3(256), 0, s, (TS *)u, lo, hi, sides); break;
+                    case 2: hipLaunchKernelGGL((k_bc_n1<TS, 2>), grid, dim3(256), 0, s, (TS *)u, lo, hi, sides); break;
+                    case 3: hipLaunchKernelGGL((k_bc_n1<TS, 3>), grid, dim3(256), 0, s, (TS *)u, lo, hi, sides); break;
+                    default: hipLaunchKernelGGL((k_bc_n1<TS, 4>), grid, dim3(256), 0, s, (TS *)u, lo, hi, sides); break;
+                    }
+                    HSX_LAUNCHED("k_bc_n1");
+                    return HSX_OK;
+                }
+                const uint32_t nbn = grid.x;
+                const dim3 both(nbn + nb1 * (uint32_t)rest.n);
+                switch ((f.he - f.hs + 1) / (16 / (int64_t)sizeof(TS))) {
+                case 1: hipLaunchKernelGGL((k_bc_n1_mirror<TS, 1>), both, dim3(256), 0, s, (TS *)u, lo, hi, sides, rest, nbn, nb1); break;
+                case 2: hipLaunchKernelGGL((k_bc_n1_mirror<TS, 2>), both, dim3(256), 0, s, (TS *)u, lo, hi, sides, rest, nbn, nb1); break;
+                case 3: hipLaunchKernelGGL((k_bc_n1_mirror<TS, 3>), both, dim3(256), 0, s, (TS *)u, lo, hi, sides, rest, nbn, nb1); break;
+                default: hipLaunchKernelGGL((k_bc_n1_mirror<TS, 4>), both, dim3(256), 0, s, (TS *)u, lo, hi, sides, rest, nbn, nb1); break;
+                }
+                HSX_LAUNCHED("k_bc_n1_mirror");
+                hipLaunchKernelGGL(k_bc_zero<TS>, dim3(nb2, rest.n), dim3(256), 0, s, (TS *)u, rest);
+                HSX_LAUNCHED("k_bc_zero");
+                return HSX_OK;
             }
             hipLaunchKernelGGL(k_bc_mirror<TS>, dim3(nb1, rest.n), dim3(256), 0, s, (TS *)u, rest);
             HSX_LAUNCHED("k_bc_mirror");

@@ -84,10 +84,12 @@ struct N1Faces {
     Face lo, hi;
     int sides;                   // 0: this grid's n1 faces stay with the generic kernels
 };
-// many grids: d_n1 per grid, nh = R / vector width (the same for all), most_rows
-// = the largest number of rows of an n1 face
-int bc_batch_n1_launch(int storage, int nshots, void *const *d_fields, const N1Faces *d_n1, int nh, int64_t most_rows,
-                       hipStream_t s);
+// the whole free surface of many grids with n1 faces in d_n1 (per grid; nh = R /
+// vector width, the same for all; most_rows = the largest number of rows of an
+// n1 face) and the other faces in d_sets: n1 rows and mirrors in one launch,
+// then the boundary planes
+int bc_batch_fs_launch(int storage, int nshots, void *const *d_fields, const N1Faces *d_n1, int nh,
+                       int64_t most_rows, const FaceSet *d_sets, int64_t most, int64_t most_plane, hipStream_t s);
 
 // ---- coefficients in the compute type (src/fdm.h:276-507, src/grid.cpp:3251-3257) --
 template <typename TC>

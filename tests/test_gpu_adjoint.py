@@ -65,6 +65,43 @@ def test_gradient_bit_exact(oracle, capi, fp64, every):
     _same(grad.download(np.float64), gref)
 
 
+@pytest.mark.parametrize("nsrc", [2, 70])
+@pytest.mark.parametrize("fp64", [False, True])
+def test_receivers_on_source_points(oracle, capi, nsrc, fp64):
+    """a whole plane of receivers (1152, several gather blocks) through which
+    the sources pass: the receivers on source points record the field after
+    the injection, as in the oracle; 2 sources take the one-launch source and
+    receiver kernel, 70 the separate inject and gather launches.  Forward
+    traces and the full-storage gradient (reverse source term and residual
+    in one launch) equal the oracle's bit for bit."""
+    print(f"receivers on sources nsrc={nsrc} fp64={fp64}", flush=True)
+    sh = make_shot(oracle, (40, 36, 32), 8, nt=24, fp64=fp64, seed=31)
+    p, r = sh["p"], sh["p"].inner()
+    i1 = r[0] + 2
+    rec = np.array([i1 + i2 * p.n1 + i3 * p.n1 * p.n2 for i3 in range(r[4], r[5] + 1)
+                    for i2 in range(r[2], r[3] + 1)], np.int64)
+    rng = np.random.default_rng(5)
+    src = np.concatenate([sh["src"], rng.choice(rec, nsrc - 1, replace=False)])
+    assert np.isin(src, rec).all() and len(np.unique(src)) == nsrc
+    wav = (sh["wavelet"] * rng.uniform(0.5, 1.5, nsrc)).astype(sh["dtype"])
+    o, coef, iv, nt = sh["o"], sh["coef"], sh["iv"], sh["nt"]
+    coef_true = sh["coef_true"]
+    d_obs = oracle.forward(o, coef_true, iv, nt, src, wav, rec)
+    ref = oracle.forward(o, coef, iv, nt, src, wav, rec)
+    g = capi.hsx_grid(p)
+    dsrc, drec, dwav, dcoef = (_dev(capi, x) for x in (src, rec, wav, coef))
+    s = capi.make_shot(nt, sh["dt"], (dsrc, nsrc), dwav, (drec, len(rec)))
+    tr = capi.DeviceArray(nt * len(rec) * np.dtype(sh["dtype"]).itemsize)
+    capi.forward(g, dcoef, s, tr)
+    _same(tr.download(sh["dtype"]).reshape(ref.shape), ref)
+    grad = _dev(capi, np.zeros(p.npoint))
+    J = capi.gradient(g, dcoef, s, _dev(capi, d_obs), grad, 0, capi.HSX_CKPT_FP32)
+    gref, Jref = oracle.gradient(o, coef, iv, sh["dt"], nt, src, wav, rec, d_obs)
+    assert J == pytest.approx(Jref, rel=1e-12)
+    assert np.abs(gref).max() > 0
+    _same(grad.download(np.float64), gref)
+
+
 @pytest.mark.parametrize("fmt,tol", [("fp16", 1e-3), ("bf16", 1e-2)])
 def test_low_precision_checkpoints_against_fp64_oracle(oracle, capi, fmt, tol):
     """FP32 stencil, 16-bit scaled checkpoints, FP64 gradient accumulation: the
