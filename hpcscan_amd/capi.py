@@ -116,6 +116,7 @@ SIGNATURES = {
     "hsx_batch_create": (_i, [_i, _G, _P(_vp), _vp, _P(_vp), _P(_vp)]),
     "hsx_batch_run": (_i, [_vp, _i, _vp]),
     "hsx_batch_autotune": (_i, [_vp, _i, _P(_i), _P(ctypes.c_float)]),
+    "hsx_batch_set_shape": (_i, [_vp, _i, _i, _i, _i]),
     "hsx_batch_destroy": (_i, [_vp]),
 }
 
@@ -436,8 +437,13 @@ class Batch:
     def run(self, use_graph=True, stream=None):
         check(lib().hsx_batch_run(self.h, 1 if use_graph else 0, stream))
 
+    def set_shape(self, lanes, rows, chunk, ring):
+        """hsx_batch_set_shape (chunk -1: one persistent launch per step)"""
+        check(lib().hsx_batch_set_shape(self.h, int(lanes), int(rows), int(chunk), int(ring)))
+
     def autotune(self, reps=3):
-        """hsx_batch_autotune: ((lanes, rows, chunk, ring), ms per step)"""
+        """hsx_batch_autotune: ((lanes, rows, chunk, ring), ms per step); chunk -1
+        is the persistent launch"""
         shape = (ctypes.c_int * 4)()
         ms = ctypes.c_float(0.0)
         check(lib().hsx_batch_autotune(self.h, int(reps), shape, ctypes.byref(ms)))

@@ -180,6 +180,103 @@ static int batch_setup_shape(hsx_batch *b, const hsx_grid *grids, const void *co
     return HSX_OK;
 }
 
+// The batched step as one persistent launch (k_tiled_batch_seg): every shot's
+// tiles as whole columns, the columns of all shots laid end to end and cut into
+// one part of equal planes per resident workgroup (at least 4R planes a part).
+template <typename TS, int R, int LX, int BY, bool ring>
+static int batch_setup_persistent(hsx_batch *b, const hsx_grid *grids, const void *const *coef)
+{
+    constexpr int VW = 16 / sizeof(TS);
+    using K = Tiled<TS, TS, R, VW, LX, BY, MODE_AC2, ring>;
+    auto kern = k_tiled_batch_seg<TS, TS, R, VW, LX, BY, MODE_AC2, ring>;
+    static const int per_cu = tiled_per_cu(kern, LX * BY, K::LDS);
+    const int ns = b->nshots;
+    const int64_t resident = (int64_t)per_cu * device_cus();
+    std::vector<TiledArgs<TS>> args[2];
+    std::vector<void *> F(2 * ns);
+    HSX_HIP(hipMemcpy(F.data(), b->fields, 2 * ns * sizeof(void *), hipMemcpyDeviceToHost));
+    for (int p = 0; p < 2; p++) args[p].resize(ns);
+    const TileCfg cfg = tile_forced();
+    int64_t planes = 0;
+    for (int s = 0; s < ns; s++) {
+        void *A = F[s], *B = F[ns + s];
+        // whole columns: one chunk of all N3 planes per tile
+        int rc = fill_tiled_args<TS, TS, R, VW, LX, BY, MODE_AC2>(&grids[s], A, A, B, coef[s], 1, cfg, args[0][s],
+                                                                 (int64_t)1 << 30);
+        if (!rc)
+            rc = fill_tiled_args<TS, TS, R, VW, LX, BY, MODE_AC2>(&grids[s], B, B, A, coef[s], 1, cfg, args[1][s],
+                                                                 (int64_t)1 << 30);
+        if (rc) return rc;
+        const TiledArgs<TS> &a = args[0][s];
+        if (a.nchunk != 1) return fail(HSX_ERR_INVALID, "batch: shot %d not in whole columns", s);
+        planes += (int64_t)a.ntx * a.nty * (a.e3 - a.s3 + 1);
+    }
+    const int64_t parts = std::max<int64_t>(1, std::min<int64_t>(resident, planes / (4 * R)));
+    if (parts > ((int64_t)1 << 30) / 8) return fail(HSX_ERR_UNSUPPORTED, "batch: too many parts");
+    // part g takes the planes [planes * g / parts, planes * (g + 1) / parts) of
+    // the columns laid end to end (shot, then tile, in order)
+    std::vector<BatchSeg> seg;
+    std::vector<int> so(parts + 1, 0);
+    int64_t pos = 0;      // first plane of the current column in the global order
+    int64_t g = 0;
+    for (int s = 0; s < ns && g < parts; s++) {
+        const TiledArgs<TS> &a = args[0][s];
+        const int64_t n3 = a.e3 - a.s3 + 1, tiles = (int64_t)a.ntx * a.nty;
+        for (int64_t t = 0; t < tiles; t++, pos += n3) {
+            // the parts that overlap [pos, pos + n3)
+            while (g < parts) {
+                const int64_t lo = planes * g / parts, hi = planes * (g + 1) / parts;
+                const int64_t a0 = std::max(lo, pos), a1 = std::min(hi, pos + n3);
+                if (a0 < a1)
+                    seg.push_back({s, (int)t, (int)(a.s3 + (a0 - pos)), (int)(a.s3 + (a1 - pos))});
+                if (hi <= pos + n3) {           // part g ends in this column: close it
+                    so[g + 1] = (int)seg.size();
+                    g++;
+                } else {
+                    break;                      // part g goes on in the next column
+                }
+            }
+        }
+    }
+    if (g != parts || (int64_t)seg.size() >= ((int64_t)1 << 31))
+        return fail(HSX_ERR_INVALID, "batch: %lld of %lld parts cut", (long long)g, (long long)parts);
+    std::vector<void *> mine;
+    auto up = [&](const void *host, size_t bytes, void **p) -> int {
+        hipError_t e = hipMalloc(p, bytes ? bytes : 1);
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc (batch step)");
+        mine.push_back(*p);
+        HSX_HIP(hipMemcpy(*p, host, bytes, hipMemcpyHostToDevice));
+        return HSX_OK;
+    };
+    void *dargs[2], *dseg, *dso;
+    int rc;
+    if ((rc = up(args[0].data(), ns * sizeof(TiledArgs<TS>), &dargs[0])) ||
+        (rc = up(args[1].data(), ns * sizeof(TiledArgs<TS>), &dargs[1])) ||
+        (rc = up(seg.data(), seg.size() * sizeof(BatchSeg), &dseg)) ||
+        (rc = up(so.data(), so.size() * sizeof(int), &dso))) {
+        for (void *q : mine) (void)hipFree(q);
+        return rc;
+    }
+    if (!b->step_owned.empty()) {
+        HSX_HIP(hipDeviceSynchronize());   // no launch of the old step may still read its tables
+        for (void *q : b->step_owned) (void)hipFree(q);
+    }
+    b->step_owned = mine;
+    b->shape[0] = LX;
+    b->shape[1] = BY;
+    b->shape[2] = -1;
+    b->shape[3] = ring ? 1 : 0;
+    const int np = (int)parts;
+    const unsigned grid = (unsigned)((parts + 7) / 8 * 8);
+    b->step = [=](int parity, hipStream_t st) -> int {
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(LX, BY), K::LDS, st, (const TiledArgs<TS> *)dargs[parity],
+                           (const BatchSeg *)dseg, (const int *)dso, np);
+        HSX_LAUNCHED("k_tiled_batch_seg");
+        return HSX_OK;
+    };
+    return HSX_OK;
+}
+
 // the fused-step instance shared by every shot: 16 rows with the halo ring
 // where its LDS fits and it keeps its values in registers, else 16 rows
 // without the ring, else 8 rows (64 lanes; 32 for FP64)
@@ -216,6 +313,10 @@ static int batch_try_shape(hsx_batch *b, int64_t chunk)
                   (RING && !Tiled<TS, TS, R, VW, LX, BY, MODE_AC2, RING>::RING)) {
         return HSX_ERR_UNSUPPORTED;
     } else {
+        if (chunk < 0) {
+            if (tiled_spills(k_tiled_batch_seg<TS, TS, R, VW, LX, BY, MODE_AC2, RING>)) return HSX_ERR_UNSUPPORTED;
+            return batch_setup_persistent<TS, R, LX, BY, RING>(b, b->grids.data(), b->coefs.data());
+        }
         if (tiled_spills(k_tiled_batch<TS, TS, R, VW, LX, BY, MODE_AC2, RING>)) return HSX_ERR_UNSUPPORTED;
         return batch_setup_shape<TS, R, LX, BY, RING>(b, b->grids.data(), b->coefs.data(), chunk);
     }
@@ -428,7 +529,7 @@ int hsx_batch_autotune(hsx_batch *b, int reps, int shape[4], float *best_ms)
     for (int lx : {64, 32})
         for (int by : {16, 8})
             for (bool ring : {true, false})
-                for (int chunk : {0, 16, 32, 64}) {
+                for (int chunk : {0, 16, 32, 64, -1}) {
                     if (by == 8 && ring) continue;
                     const Cand c = {lx, by, ring, chunk};
                     if (c.lx == cand[0].lx && c.by == cand[0].by && c.ring == cand[0].ring && c.chunk == cand[0].chunk)
@@ -472,6 +573,30 @@ int hsx_batch_autotune(hsx_batch *b, int reps, int shape[4], float *best_ms)
     if (shape)
         for (int k = 0; k < 4; k++) shape[k] = b->shape[k];
     if (best_ms) *best_ms = best_t;
+    return HSX_OK;
+}
+
+int hsx_batch_set_shape(hsx_batch *b, int lanes, int rows, int chunk, int ring)
+{
+    if (!b) return fail(HSX_ERR_INVALID, "null batch");
+    if (chunk < -1 || chunk > (1 << 20)) return fail(HSX_ERR_INVALID, "batch shape: chunk %d", chunk);
+    int rc = ensure_init();
+    if (rc) return rc;
+    const int R = b->grids[0].fd_order / 2;
+    rc = dispatch_R(R, [&](auto rc_) -> int {
+        constexpr int RR = decltype(rc_)::value;
+        if (b->storage == HSX_FP64) return batch_shape_rt<double, RR>(b, lanes, rows, ring != 0, chunk);
+        return batch_shape_rt<float, RR>(b, lanes, rows, ring != 0, chunk);
+    });
+    if (rc == HSX_ERR_UNSUPPORTED)
+        return fail(HSX_ERR_UNSUPPORTED, "batch shape %d x %d ring %d does not fit this order and precision", lanes,
+                    rows, ring);
+    if (rc) return rc;
+    // a captured time loop launched the old step: capture again
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
     return HSX_OK;
 }
 

@@ -167,9 +167,9 @@ __device__ __forceinline__ int xcd_logical(int total)
     return (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
 }
 
-// one workgroup: tile / chunk number 'logical' of the grid described by a
+// one workgroup: planes [z0, z1) of tile 'tile' of the grid described by a
 template <typename TS, typename TC, int R, int VW, int LX, int BY, int MODE, bool RING_, bool FOLD = false>
-__device__ __forceinline__ void tiled_body(const TiledArgs<TC> &a, int logical)
+__device__ __forceinline__ void tiled_body_tz(const TiledArgs<TC> &a, int tile, int64_t z0, int64_t z1)
 {
     using K = Tiled<TS, TC, R, VW, LX, BY, MODE, RING_>;
     using V = typename K::V;
@@ -181,16 +181,12 @@ __device__ __forceinline__ void tiled_body(const TiledArgs<TC> &a, int logical)
     V *ring_h = reinterpret_cast<V *>(hsx_lds_raw + K::PLANES_BYTES);
     V *ring_c = reinterpret_cast<V *>(hsx_lds_raw + K::PLANES_BYTES + K::RING_H);
 
-    const int tile = logical % (a.ntx * a.nty);
-    const int chunk = logical / (a.ntx * a.nty);
     const int tx = threadIdx.x, ty = threadIdx.y;
 
     const int64_t t1 = a.t1s + (int64_t)(tile % a.ntx) * (LX * VW);            // first n1 index of the tile
     const int64_t i1 = t1 + tx * VW;                                           // first n1 index of this lane
     const int64_t t2s = a.s2 + (int64_t)(tile / a.ntx) * BY;                   // first n2 row of the tile
     const int64_t i2 = t2s + ty;
-    const int64_t z0 = a.s3 + (int64_t)chunk * a.chunk;
-    const int64_t z1 = min(z0 + a.chunk, a.e3 + 1);    // planes [z0, z1)
 
     const bool lane_in_row = i1 + VW - 1 < a.n1;        // vector fully inside the row
     const bool row_ok = i2 < a.n2;
@@ -653,6 +649,17 @@ __device__ __forceinline__ void tiled_body(const TiledArgs<TC> &a, int logical)
     }
 }
 
+// one workgroup: tile / chunk number 'logical' of the grid described by a
+template <typename TS, typename TC, int R, int VW, int LX, int BY, int MODE, bool RING_, bool FOLD = false>
+__device__ __forceinline__ void tiled_body(const TiledArgs<TC> &a, int logical)
+{
+    const int tile = logical % (a.ntx * a.nty);
+    const int chunk = logical / (a.ntx * a.nty);
+    const int64_t z0 = a.s3 + (int64_t)chunk * a.chunk;
+    const int64_t z1 = min(z0 + a.chunk, a.e3 + 1);    // planes [z0, z1)
+    tiled_body_tz<TS, TC, R, VW, LX, BY, MODE, RING_, FOLD>(a, tile, z0, z1);
+}
+
 template <typename TS, typename TC, int R, int VW, int LX, int BY, int MODE, bool RING_, bool FOLD = false>
 __global__ void __launch_bounds__(LX *BY) k_tiled(TiledArgs<TC> a)
 {
@@ -676,6 +683,33 @@ __global__ void __launch_bounds__(LX *BY) k_tiled_batch(const TiledArgs<TC> *__r
     while (s + 1 < nshots && off[s + 1] <= logical) s++;
     const TiledArgs<TC> a = grids[s];
     tiled_body<TS, TC, R, VW, LX, BY, MODE, RING_>(a, logical - off[s]);
+}
+
+// The batched step as one persistent launch: as many workgroups as the device
+// holds at once, each running its own list of column segments seg[so[g]] ..
+// seg[so[g + 1] - 1] (shot, tile, planes [z0, z1)) one after the other.  The
+// host cuts the columns of all shots, laid end to end, into equal numbers of
+// planes per workgroup, so that a workgroup streams one or two long segments
+// (2R planes of queue priming each) instead of many short chunks, and the
+// launch has no second round.  Consecutive parts go to workgroups of one XCD
+// (xcd_logical), so that tiles sharing halo rows share an L2.
+struct BatchSeg {
+    int shot, tile;
+    int z0, z1;
+};
+
+template <typename TS, typename TC, int R, int VW, int LX, int BY, int MODE, bool RING_>
+__global__ void __launch_bounds__(LX *BY) k_tiled_batch_seg(const TiledArgs<TC> *__restrict__ grids,
+                                                          const BatchSeg *__restrict__ seg, const int *__restrict__ so,
+                                                          int parts)
+{
+    const int part = xcd_logical(parts);
+    if (part >= parts) return;                          // whole block leaves, before any barrier
+    for (int k = so[part]; k < so[part + 1]; k++) {
+        const BatchSeg sg = seg[k];
+        if (k > so[part]) __syncthreads();              // the previous segment's last plane is read
+        tiled_body_tz<TS, TC, R, VW, LX, BY, MODE, RING_>(grids[sg.shot], sg.tile, sg.z0, sg.z1);
+    }
 }
 
 // ---- launch configuration -------------------------------------------------------------------

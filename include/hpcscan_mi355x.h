@@ -316,13 +316,20 @@ int hsx_batch_create(int nshots, const hsx_grid *grids, const void *const *coef,
 int hsx_batch_run(hsx_batch *b, int use_graph, void *stream);
 /* Tile autotuner of a batch (no counterpart in the reference): times the
  * batched fused step with each candidate shape -- lanes x rows (64 / 32 x 16 / 8),
- * halo ring on / off, n3 chunk (the rule, 16, 32, 64 planes) -- for reps
- * launches, starting with the shape hsx_batch_create chose, and keeps the
- * fastest for later runs (a captured time loop is captured again).
- * shape[4] (may be NULL) = lanes, rows, chunk (0: the rule), ring of the
- * choice; *best_ms (may be NULL) its time per step.  The wavefields are
- * overwritten (every run starts from zero anyway). */
+ * halo ring on / off, n3 chunk (the rule, 16, 32, 64 planes, or -1: one
+ * persistent launch, each resident workgroup streaming an equal share of the
+ * shots' whole tile columns) -- for reps launches, starting with the shape
+ * hsx_batch_create chose, and keeps the fastest for later runs (a captured
+ * time loop is captured again).  shape[4] (may be NULL) = lanes, rows, chunk
+ * (0: the rule, -1: persistent), ring of the choice; *best_ms (may be NULL)
+ * its time per step.  The wavefields are overwritten (every run starts from
+ * zero anyway). */
 int hsx_batch_autotune(hsx_batch *b, int reps, int shape[4], float *best_ms);
+/* The batched step with a given shape (the values hsx_batch_autotune reports):
+ * HSX_ERR_UNSUPPORTED when it does not fit this order and precision (LDS,
+ * registers); a captured time loop is captured again.  No counterpart in the
+ * reference. */
+int hsx_batch_set_shape(hsx_batch *b, int lanes, int rows, int chunk, int ring);
 int hsx_batch_destroy(hsx_batch *b);
 
 #ifdef __cplusplus

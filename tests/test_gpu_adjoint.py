@@ -377,6 +377,42 @@ def test_batch_autotune_keeps_results(oracle, capi, fp64):
     b.close()
 
 
+@pytest.mark.parametrize("fp64", [False, True])
+def test_batch_persistent_step_bit_exact(oracle, capi, fp64):
+    """the batched step as one persistent launch (chunk -1: each resident
+    workgroup streams an equal share of the shots' whole tile columns, cut
+    across column ends) with two tile shapes each (FP32: 64 x 16 ring, 32 x 8;
+    FP64: 32 x 8, 64 x 8), set by
+    hsx_batch_set_shape, and a batch with far more columns than workgroups
+    (16 shots of 800 rows): traces equal the oracle's, with and without the
+    graph; a shape that does not exist is refused"""
+    for sizes, nt in (([(36, 20, 18), (68, 36, 20), (40, 24, 22)], 24),
+                      ([(16 + 4 * (k % 3), 800, 36 + 2 * k) for k in range(16)], 10)):
+        shots = []
+        for k, n in enumerate(sizes):
+            sh = make_shot(oracle, n, 8, nt=nt, fp64=fp64, seed=130 + k, with_obs=False)
+            d, s = _shot_dev(capi, sh)
+            tr = capi.DeviceArray(sh["nt"] * len(sh["rec"]) * np.dtype(sh["dtype"]).itemsize)
+            shots.append((sh, d, s, tr))
+        refs = [oracle.forward(sh["o"], sh["coef"], sh["iv"], sh["nt"], sh["src"], sh["wavelet"], sh["rec"])
+                for sh, _, _, _ in shots]
+        b = capi.Batch([capi.hsx_grid(x[0]["p"]) for x in shots], [x[1]["coef"] for x in shots],
+                       [x[2] for x in shots], [x[3] for x in shots])
+        shapes = [(32, 8, -1, 0), (64, 8, -1, 0)] if fp64 else [(64, 16, -1, 1), (32, 8, -1, 0)]
+        for shape in shapes:
+            print(f"persistent batch fp64={fp64} shots={len(sizes)} shape {shape}", flush=True)
+            b.set_shape(*shape)
+            for use_graph in (True, False):
+                for sh, _, _, tr in shots:        # stale traces would show as NaN
+                    tr.upload(np.full(tr.nbytes // np.dtype(sh["dtype"]).itemsize, np.nan, sh["dtype"]))
+                b.run(use_graph=use_graph)
+                for (sh, d, s, tr), ref in zip(shots, refs):
+                    _same(tr.download(sh["dtype"]).reshape(ref.shape), ref)
+        with pytest.raises(capi.HsxError):
+            b.set_shape(48, 8, -1, 0)
+        b.close()
+
+
 def test_plan_with_autotune_bit_exact(oracle, capi):
     """a plan made with HSX_ADJ_AUTOTUNE (the layout's tile shape tuned first)
     gives the oracle's gradient bit for bit"""
