@@ -220,6 +220,9 @@ def gpu_leg(args):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "npt": npt, "device": info, "dt": float(dt), "tuned": tuned,
+        "default_path": default_shape and not args.fold and not args.contract and args.kernel == "auto"
+                        and (tuned is None or (tuned["lanes"], tuned["rows"], tuned["chunk"], tuned["flags"]) == (0, 0, 0, 0)
+                             or (tuned["lanes"], tuned["rows"], tuned["chunk"], tuned["flags"]) == (64, 16, 0, 0)),
         "l1_last": {"step": it_last, "time_s": t_last, "l1_vs_analytical": l1_last},
         "pointwise_ref": None if pw_ms is None else {
             "kernel": "k_update_pressure16", "bytes_per_point": 20, "ms": pw_ms,
@@ -289,6 +292,66 @@ def run_harness(exe, mode, n, order, nt, ntry, env=None, timeout=900):
     if val is None:
         raise RuntimeError(f"no 'Best Gpoint/s eff.' line in the output of {' '.join(cmd)}")
     return val, err, wall, " ".join(cmd[1:])
+
+
+def traffic_leg(args):
+    """HBM bytes per fused-step launch, measured in this run: two rocprofv3
+    --pmc passes (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch) over a short
+    child run of this bench (same grid and shape, 1 + 3 steps, --calibrate-copy),
+    each scaled by what the same counter reads for the plain copy of one whole
+    grid (npoint x 4 bytes read and written) in that run.  Returns None (the
+    committed figure is used) when rocprofv3 gives no counter file."""
+    import collections
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+
+    from hpcscan_amd.layout import make_layout
+
+    p = make_layout(3, (args.n, args.n, args.n), args.order)
+    copy_kib = p.npoint * 4 / 1024.0
+    child = [sys.executable, os.path.abspath(__file__), "--steps", "3", "--warmup", "1", "--n", str(args.n),
+             "--order", str(args.order), "--kernel", args.kernel, "--no-cpu-baseline", "--no-adjoint", "--no-batch",
+             "--no-autotune", "--no-analytical", "--no-pointwise-ref", "--no-traffic", "--calibrate-copy"]
+    got = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="hsx_pmc_", dir="/tmp")
+        try:
+            cmd = ["rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "p", "--"] + child
+            t0 = time.perf_counter()
+            try:
+                r = subprocess.run(cmd, capture_output=True, text=True, timeout=150, cwd="/tmp",
+                                   env=dict(os.environ, TMPDIR="/tmp"))
+                rc = r.returncode
+            except subprocess.TimeoutExpired:
+                rc = "timeout"
+            f = os.path.join(d, "p_counter_collection.csv")
+            if not os.path.exists(f):
+                log(f"traffic: rocprofv3 --pmc {counter} left no counter file ({rc}); using the committed figure")
+                return None
+            vals = collections.defaultdict(list)
+            for row in csv.DictReader(open(f)):
+                if row.get("Counter_Name") != counter:
+                    continue
+                k = row["Kernel_Name"]
+                if "k_tiled<" in k:
+                    vals["step"].append(float(row["Counter_Value"]))
+                elif "k_array16" in k:
+                    vals["copy"].append(float(row["Counter_Value"]))
+            if not vals["step"] or len(vals["copy"]) != 1 or vals["copy"][0] <= 0:
+                log(f"traffic: {counter} rows missing (step {len(vals['step'])}, copy {len(vals['copy'])})")
+                return None
+            got[counter] = (sum(vals["step"]) / len(vals["step"]), vals["copy"][0])
+            log(f"traffic: {counter} {got[counter][0]:.1f} KiB per step launch, copy {got[counter][1]:.1f} KiB "
+                f"({time.perf_counter() - t0:.0f} s pass)")
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+    (fs, fc), (ws, wc) = got["FETCH_SIZE"], got["WRITE_SIZE"]
+    fscale, wscale = copy_kib / fc, copy_kib / wc
+    return {"bytes": (fs * fscale + ws * wscale) * 1024.0, "read_bytes": fs * fscale * 1024.0,
+            "write_bytes": ws * wscale * 1024.0, "fetch_scale": round(fscale, 4), "write_scale": round(wscale, 4),
+            "source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE in this run, scaled by a whole-grid copy"}
 
 
 def analytical_leg(args, cpu, gpu):
@@ -603,6 +666,9 @@ def main():
                     help="fused multiply-adds in the fused step (hsx_set_contract): not the reference's bits")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the library's default tile shape (no hsx_autotune before timing)")
+    ap.add_argument("--no-traffic", dest="traffic", action="store_false",
+                    help="take roofline.traffic from the committed counter summary instead of measuring it with "
+                         "two rocprofv3 --pmc child runs")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one full-grid copy kernel first (counter calibration for profiles)")
     ap.add_argument("--no-batch", dest="batch", action="store_false",
@@ -637,6 +703,17 @@ def main():
     adj = None if args.no_adjoint else adjoint_leg(args)
     bat = batch_leg(args) if args.batch else None
     cpu = None if args.no_cpu_baseline else cpu_leg(args)
+    # the child run takes the library's default shape and arithmetic: measured
+    # only when this run used them too
+    if args.traffic and gpu["default_path"]:
+        tr = None
+        try:
+            tr = traffic_leg(args)
+        except Exception as e:
+            log(f"traffic: measurement failed ({e}); using the committed figure")
+        if tr is not None:
+            gpu["roofline"]["traffic"] = tr["bytes"]
+            gpu["roofline"]["traffic_measured"] = tr
     ana = None
     if not args.no_analytical:
         try:
