@@ -56,7 +56,6 @@ __global__ void k_sum_partials(const double *partial, int n, double *out)
 // (the membership test is a loop over the sources).
 constexpr int64_t SRCREC_MAX = 64;
 
-template <typename TS>
 __device__ __forceinline__ bool on_source(int64_t j, int64_t nsrc, const int64_t *src)
 {
     for (int64_t k = 0; k < nsrc; k++)
@@ -77,14 +76,14 @@ __global__ void __launch_bounds__(256) k_srcrec(TS *u, int64_t npoint, int64_t n
         __syncthreads();
         for (int64_t i = threadIdx.x; i < nrec; i += blockDim.x) {
             const int64_t j = rec[i];
-            if (on_source<TS>(j, nsrc, src)) tr[i] = (uint64_t)j < (uint64_t)npoint ? u[j] : (TS)0;
+            if (on_source(j, nsrc, src)) tr[i] = (uint64_t)j < (uint64_t)npoint ? u[j] : (TS)0;
         }
         return;
     }
     for (int64_t i = (int64_t)(blockIdx.x - 1) * blockDim.x + threadIdx.x; i < nrec;
          i += (int64_t)(gridDim.x - 1) * blockDim.x) {
         const int64_t j = rec[i];
-        if (!on_source<TS>(j, nsrc, src)) tr[i] = (uint64_t)j < (uint64_t)npoint ? u[j] : (TS)0;
+        if (!on_source(j, nsrc, src)) tr[i] = (uint64_t)j < (uint64_t)npoint ? u[j] : (TS)0;
     }
 }
 
