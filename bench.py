@@ -204,8 +204,18 @@ def gpu_leg(args):
     alg_bytes = 16 * npt
     achieved = alg_bytes / avg_kernel_s / 1e9
     traffic = None
-    default_shape = (args.tile_lx, args.tile_rows, args.tile_chunk, args.streams, args.no_ring) == \
-        (0, 0, 0, "default", False)
+    # the shape the timed launches used (forced, else tuned), as command-line
+    # arguments for the child runs that count its HBM bytes
+    if tuned is not None:
+        lx, by, ch, fl = tuned["lanes"], tuned["rows"], tuned["chunk"], tuned["flags"]
+        streams = "cached" if fl & capi.HSX_TILE_CACHED else "nt" if fl & capi.HSX_TILE_NT else "default"
+        no_ring = bool(fl & capi.HSX_TILE_NO_RING)
+    else:
+        lx, by, ch, streams, no_ring = args.tile_lx, args.tile_rows, args.tile_chunk, args.streams, args.no_ring
+    shape_args = ["--tile-lx", str(lx), "--tile-rows", str(by), "--tile-chunk", str(ch), "--streams", streams] + \
+        (["--no-ring"] if no_ring else [])
+    # the default shape at 1000^3 order 8 is 64 x 16 with the ring
+    default_shape = (lx, by, ch, streams, no_ring) in ((0, 0, 0, "default", False), (64, 16, 0, "default", False))
     if os.path.exists(TRAFFIC_FILE) and (n, order, args.kernel) == (1000, 8, "auto") and default_shape:
         try:
             tf = json.load(open(TRAFFIC_FILE))
@@ -220,9 +230,7 @@ def gpu_leg(args):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "npt": npt, "device": info, "dt": float(dt), "tuned": tuned,
-        "default_path": default_shape and not args.fold and not args.contract and args.kernel == "auto"
-                        and (tuned is None or (tuned["lanes"], tuned["rows"], tuned["chunk"], tuned["flags"]) == (0, 0, 0, 0)
-                             or (tuned["lanes"], tuned["rows"], tuned["chunk"], tuned["flags"]) == (64, 16, 0, 0)),
+        "countable": not args.fold and not args.contract and args.kernel == "auto", "shape_args": shape_args,
         "l1_last": {"step": it_last, "time_s": t_last, "l1_vs_analytical": l1_last},
         "pointwise_ref": None if pw_ms is None else {
             "kernel": "k_update_pressure16", "bytes_per_point": 20, "ms": pw_ms,
@@ -294,13 +302,14 @@ def run_harness(exe, mode, n, order, nt, ntry, env=None, timeout=900):
     return val, err, wall, " ".join(cmd[1:])
 
 
-def traffic_leg(args):
+def traffic_leg(args, shape_args):
     """HBM bytes per fused-step launch, measured in this run: two rocprofv3
     --pmc passes (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch) over a short
     child run of this bench (same grid and shape, 1 + 3 steps, --calibrate-copy),
     each scaled by what the same counter reads for the plain copy of one whole
-    grid (npoint x 4 bytes read and written) in that run.  Returns None (the
-    committed figure is used) when rocprofv3 gives no counter file."""
+    grid (npoint x 4 bytes read and written) in that run; shape_args repeat the
+    tile shape the timed launches used.  Returns None (the committed figure is
+    used) when rocprofv3 gives no counter file."""
     import collections
     import csv
     import shutil
@@ -313,7 +322,7 @@ def traffic_leg(args):
     copy_kib = p.npoint * 4 / 1024.0
     child = [sys.executable, os.path.abspath(__file__), "--steps", "3", "--warmup", "1", "--n", str(args.n),
              "--order", str(args.order), "--kernel", args.kernel, "--no-cpu-baseline", "--no-adjoint", "--no-batch",
-             "--no-autotune", "--no-analytical", "--no-pointwise-ref", "--no-traffic", "--calibrate-copy"]
+             "--no-autotune", "--no-analytical", "--no-pointwise-ref", "--no-traffic", "--calibrate-copy"] + shape_args
     got = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         d = tempfile.mkdtemp(prefix="hsx_pmc_", dir="/tmp")
@@ -650,7 +659,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-analytical", action="store_true",
                     help="skip the MI355X harness run that reports errTestCase2 on the CPU baseline's command line")
-    ap.add_argument("--tile-lx", type=int, default=0, choices=[0, 16, 32, 64],
+    ap.add_argument("--tile-lx", type=int, default=0, choices=[0, 16, 32, 64, 128],
                     help="lanes along n1 per tile of the fused kernel (0 = chosen from the grid width)")
     ap.add_argument("--tile-rows", type=int, default=0, choices=[0, 8, 16],
                     help="rows along n2 per tile (0 = library choice)")
@@ -703,12 +712,12 @@ def main():
     adj = None if args.no_adjoint else adjoint_leg(args)
     bat = batch_leg(args) if args.batch else None
     cpu = None if args.no_cpu_baseline else cpu_leg(args)
-    # the child run takes the library's default shape and arithmetic: measured
-    # only when this run used them too
-    if args.traffic and gpu["default_path"]:
+    # the child run repeats this run's tile shape; the folded free surface and the
+    # contracted arithmetic are not counted
+    if args.traffic and gpu["countable"]:
         tr = None
         try:
-            tr = traffic_leg(args)
+            tr = traffic_leg(args, gpu["shape_args"])
         except Exception as e:
             log(f"traffic: measurement failed ({e}); using the committed figure")
         if tr is not None:

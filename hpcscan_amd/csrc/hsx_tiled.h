@@ -976,7 +976,7 @@ bool tuned_shape(const hsx_grid *g, int mode, TileCfg *out);
 
 template <int MODE, bool FOLD = false>
 inline int run_tiled_shape(const hsx_grid *g, void *out, const void *prn_in, const void *u, const void *coef,
-                           hipStream_t s, const TileCfg &cfg)
+                           hipStream_t s, const TileCfg &cfg_in)
 {
     const int R = g->fd_order / 2;
     return dispatch_types(g, [&](auto ts, auto tc) -> int {
@@ -987,9 +987,28 @@ inline int run_tiled_shape(const hsx_grid *g, void *out, const void *prn_in, con
         if (rc) return rc;
         if (((uintptr_t)out | (uintptr_t)prn_in | (uintptr_t)u | (uintptr_t)coef) % 16)
             return fail(HSX_ERR_UNSUPPORTED, "buffers not 16-byte aligned");
-        const int lx = pick_lx(g, VW, cfg);
+        TileCfg cfg = cfg_in;
+        int lx = pick_lx(g, VW, cfg);
         return dispatch_R(R, [&](auto rc_) -> int {
             constexpr int RR = decltype(rc_)::value;
+            if (lx == 128) {
+                // 128 lanes x 8 rows with the halo ring (rows of 2 KiB, two
+                // waves each): the exact FP32 fused step and Laplacian up to
+                // order 8, where the ring's LDS fits and nothing spills; on
+                // request only (hsx_set_tile, the autotuner). 1000^3 order 8:
+                // 3.04-3.08 ms against 3.12-3.19 ms for 64 x 16; 500^3 0.363
+                // against 0.394 ms; 700^3 and 800^3 6 % and 2 % slower.
+                using K = Tiled<TS, TC, RR, VW, 128, 8, MODE, true>;
+                if constexpr (sizeof(TS) == 4 && sizeof(TC) == 4 && !FOLD && (MODE == MODE_AC2 || MODE == MODE_LAP) &&
+                              K::RING && K::LDS <= TILE_LDS_MAX) {
+                    static const bool spills = tiled_spills(k_tiled<TS, TC, RR, VW, 128, 8, MODE, true>);
+                    if (!spills && !(cfg.flags & HSX_TILE_NO_RING))
+                        return launch_tiled<TS, TC, RR, VW, 128, 8, MODE, true>(g, out, prn_in, u, coef, s, cfg);
+                }
+                cfg.lx = 0;     // elsewhere: the library's choice of lanes and rows
+                cfg.by = 0;
+                lx = pick_lx(g, VW, cfg);
+            }
             if (lx == 16)
                 return launch_tiled<TS, TC, RR, VW, 16, 8, MODE, false, FOLD>(g, out, prn_in, u, coef, s, cfg);
             // 64 lanes without a 16-row instance that keeps its values in

@@ -20,9 +20,10 @@ TileCfg tile_forced(void)
 
 int set_tile(int lx, int by, int chunk, int flags)
 {
-    if (lx != 0 && lx != 16 && lx != 32 && lx != 64)
-        return fail(HSX_ERR_INVALID, "tile lanes %d not in {0,16,32,64}", lx);
+    if (lx != 0 && lx != 16 && lx != 32 && lx != 64 && lx != 128)
+        return fail(HSX_ERR_INVALID, "tile lanes %d not in {0,16,32,64,128}", lx);
     if (by != 0 && by != 8 && by != 16) return fail(HSX_ERR_INVALID, "tile rows %d not in {0,8,16}", by);
+    if (lx == 128 && by == 16) return fail(HSX_ERR_INVALID, "128-lane tiles have 8 rows (1024 threads)");
     if (chunk < 0) return fail(HSX_ERR_INVALID, "chunk %d < 0", chunk);
     if (flags & ~(HSX_TILE_CACHED | HSX_TILE_NT | HSX_TILE_NO_RING))
         return fail(HSX_ERR_INVALID, "tile flags 0x%x", flags);
@@ -123,6 +124,7 @@ extern "C" int hsx_autotune(const hsx_grid *g, int reps, int shape[4], float *be
         {64, 8, 0, 0}, {64, 8, 0, HSX_TILE_NO_RING}, {64, 8, 0, HSX_TILE_NO_RING | HSX_TILE_NT},
         {64, 8, (int)std::min<int64_t>(N3, 8 * R + 64), HSX_TILE_NO_RING},
         {32, 8, 0, 0}, {32, 8, 0, HSX_TILE_NO_RING}, {16, 8, 0, 0},
+        {128, 8, 0, 0},
     };
     TileCfg best = {0, 0, 0, 0};
     float best_t = 1e30f;

@@ -107,7 +107,10 @@ int         hsx_get_kernel(void);
 int         hsx_set_contract(int on);
 int         hsx_get_contract(void);
 /* Tile shape of the tiled kernels (0 = the library's choice for each):
- *   lanes_n1      16, 32 or 64 lanes of 16 bytes along n1 (default by grid width)
+ *   lanes_n1      16, 32 or 64 lanes of 16 bytes along n1 (default by grid width),
+ *                 or 128 (8 rows with the halo ring: the exact FP32 fused step
+ *                 and Laplacian up to order 8; the other kernels, precisions and
+ *                 orders then keep the library's choice)
  *   rows_n2       8 or 16 rows along n2 (16 needs lanes_n1 >= 32 and its LDS
  *                 planes within the 160 KiB of a workgroup; otherwise 8)
  *   chunk_planes  n3 planes streamed by one workgroup

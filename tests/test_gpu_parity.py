@@ -392,6 +392,15 @@ TILED_CASES = [
     (12, "f64", 66, 30, 26, 32, 11, 0, 1),
     (8, "f32c64", 72, 18, 21, 16, 5, 0, 0),   # FP32 storage, FP64 compute
     (8, "f32c64", 140, 35, 21, 64, 5, 16, 2),
+    (8, "f32", 600, 20, 30, 128, 0, 8, 0),     # 128 x 8 ring: two tiles of 512 points, the second partial
+    (8, "f32", 300, 21, 19, 128, 0, 0, 0),     # 128 lanes, rows left to the library (8)
+    (8, "f32", 560, 21, 40, 128, 7, 8, 0),     # 128 x 8 ring in 7-plane chunks
+    (6, "f32", 520, 19, 23, 128, 0, 8, 0),     # R = 3 (n1 offset)
+    (2, "f32", 700, 17, 20, 128, 0, 8, 2),     # R = 1, nontemporal streams
+    (4, "f32", 520, 20, 20, 128, 0, 8, 0),     # R = 2: the instance would spill, library choice instead
+    (16, "f32", 260, 40, 36, 128, 0, 8, 0),    # R = 8: ring too large, library choice
+    (8, "f64", 300, 20, 22, 128, 0, 8, 0),     # FP64: library choice
+    (8, "f32", 560, 21, 40, 128, 0, 8, 4),     # 128 lanes without the ring: library choice
 ]
 
 
@@ -440,7 +449,8 @@ def test_tiled_shapes_bit_exact(oracle, capi, order, prec, n1, n2, n3, lanes, ch
         capi.set_kernel(capi.HSX_KERNEL_AUTO)
 
 
-@pytest.mark.parametrize("args", [(8, 0, 0, 0), (64, -1, 0, 0), (64, 0, 12, 0), (64, 0, 0, 3), (64, 0, 0, 8)])
+@pytest.mark.parametrize("args", [(8, 0, 0, 0), (64, -1, 0, 0), (64, 0, 12, 0), (64, 0, 0, 3), (64, 0, 0, 8),
+                                  (128, 0, 16, 0), (256, 0, 8, 0)])
 def test_set_tile_rejects_bad_shapes(capi, args):
     lanes, chunk, rows, flags = args
     with pytest.raises(capi.HsxError):
