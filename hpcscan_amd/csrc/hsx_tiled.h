@@ -994,8 +994,9 @@ inline int run_tiled_shape(const hsx_grid *g, void *out, const void *prn_in, con
             if (lx == 128) {
                 // 128 lanes x 8 rows with the halo ring (rows of 2 KiB, two
                 // waves each): the exact FP32 fused step and Laplacian up to
-                // order 8, where the ring's LDS fits and nothing spills; on
-                // request only (hsx_set_tile, the autotuner). 1000^3 order 8:
+                // order 8, where the ring's LDS fits and spills stay within
+                // TILED_SPILL_OK; on request only (hsx_set_tile, the
+                // autotuner). 1000^3 order 8:
                 // 3.04-3.08 ms against 3.12-3.19 ms for 64 x 16; 500^3 0.363
                 // against 0.394 ms; 700^3 and 800^3 6 % and 2 % slower.
                 using K = Tiled<TS, TC, RR, VW, 128, 8, MODE, true>;
@@ -1028,6 +1029,23 @@ inline int run_tiled_shape(const hsx_grid *g, void *out, const void *prn_in, con
             return launch_tiled_by<TS, TC, RR, VW, 64, MODE, FOLD>(g, out, prn_in, u, coef, s, cfg);
         });
     });
+}
+
+// whether the 128 x 8 tiles exist for g (FP32, the ring's LDS fits, no spill):
+// elsewhere a request for them runs the library's choice
+inline bool wide_tiles_fit(const hsx_grid *g)
+{
+    if (g->storage != HSX_FP32 || g->compute != HSX_FP32 || g->dim != 3) return false;
+    return dispatch_R(g->fd_order / 2, [&](auto rc_) -> int {
+               constexpr int RR = decltype(rc_)::value;
+               using K = Tiled<float, float, RR, 4, 128, 8, MODE_AC2, true>;
+               if constexpr (K::RING && K::LDS <= TILE_LDS_MAX) {
+                   static const bool spills = tiled_spills(k_tiled<float, float, RR, 4, 128, 8, MODE_AC2, true>);
+                   return spills ? 0 : 1;
+               } else {
+                   return 0;
+               }
+           }) == 1;
 }
 
 // the forced shape (hsx_set_tile) if any, else the tuned one, else the defaults

@@ -126,8 +126,7 @@ int         hsx_set_tile(int lanes_n1, int rows_n2, int chunk_planes, int flags)
 /* Tile autotuner (no counterpart in the reference): times the candidate
  * shapes of the fused step on scratch grids of g's layout (reps launches each)
  * and keeps the fastest for g's layout, FD order and precisions; the fused
- * step and the Laplacian on such grids then use it unless hsx_set_tile forces
- * a shape.  shape[4] (may be NULL) = lanes, rows, chunk, flags of the choice;
+ * step on such grids then uses it unless hsx_set_tile forces a shape.  shape[4] (may be NULL) = lanes, rows, chunk, flags of the choice;
  * *best_ms (may be NULL) its time per launch.  Allocates 3 grids while it runs. */
 int         hsx_autotune(const hsx_grid *g, int reps, int shape[4], float *best_ms);
 int         hsx_autotune_clear(void);

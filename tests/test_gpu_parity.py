@@ -556,8 +556,9 @@ def _step_ms(capi, g, dp, du, dc, reps):
                                           ((256, 240, 200), 8, "f64")])
 def test_autotuned_shape_bit_exact_and_not_slower(oracle, capi, n, order, prec):
     """hsx_autotune picks a tile shape for the layout (its sweep includes the
-    library's default shape); the fused step and the Laplacian then run with
-    it, bit for bit as the oracle, in no more time than the default shape"""
+    library's default shape); the fused step then runs with it, bit for bit as
+    the oracle, in no more time than the default shape (the Laplacian keeps
+    its own shape and stays exact)"""
     dtype, c64 = PRECS[prec]
     p, o = _layouts(oracle, 3, n, order, dtype)
     g = capi.hsx_grid(p, compute_fp64=c64)
