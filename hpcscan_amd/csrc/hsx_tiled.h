@@ -974,6 +974,36 @@ inline int tiled_fits(const hsx_grid *g, int vw_elems, size_t elem)
 // shape chosen by hsx_autotune for this grid and operation, if any
 bool tuned_shape(const hsx_grid *g, int mode, TileCfg *out);
 
+// whether the 128 x 8 tiles exist for g (FP32, the ring's LDS fits, spills
+// within TILED_SPILL_OK): elsewhere a request for them runs the library's choice
+inline bool wide_tiles_fit(const hsx_grid *g)
+{
+    if (g->storage != HSX_FP32 || g->compute != HSX_FP32 || g->dim != 3) return false;
+    return dispatch_R(g->fd_order / 2, [&](auto rc_) -> int {
+               constexpr int RR = decltype(rc_)::value;
+               using K = Tiled<float, float, RR, 4, 128, 8, MODE_AC2, true>;
+               if constexpr (K::RING && K::LDS <= TILE_LDS_MAX) {
+                   static const bool spills = tiled_spills(k_tiled<float, float, RR, 4, 128, 8, MODE_AC2, true>);
+                   return spills ? 0 : 1;
+               } else {
+                   return 0;
+               }
+           }) == 1;
+}
+
+// The fused step's own choice of 128 x 8 tiles (with the ring) over 64 x 16:
+// where both span the same width and either one 512-point tile covers the row
+// or the row fills at least 93 % of the 128-lane tiles. Order 8 FP32, median
+// of 48 launches each on one device (128 x 8 / 64 x 16): 400^3 0.958, 460^3
+// 0.894, 500^3 0.911, 512^3 0.967, 960^3 0.992, 1000^3 0.966, 1020^3 0.988;
+// not taken at 600^3 1.202, 800^3 1.032, 900^3 1.016, 1100^3 1.053.
+inline bool wide_by_default(const hsx_grid *g)
+{
+    const int64_t N1 = g->inner[1] - g->inner[0] + 1;
+    const int64_t w64 = (N1 + 255) / 256 * 256, w128 = (N1 + 511) / 512 * 512;
+    return w64 == w128 && (w128 == 512 || 100 * N1 >= 93 * w128);
+}
+
 template <int MODE, bool FOLD = false>
 inline int run_tiled_shape(const hsx_grid *g, void *out, const void *prn_in, const void *u, const void *coef,
                            hipStream_t s, const TileCfg &cfg_in)
@@ -989,6 +1019,9 @@ inline int run_tiled_shape(const hsx_grid *g, void *out, const void *prn_in, con
             return fail(HSX_ERR_UNSUPPORTED, "buffers not 16-byte aligned");
         TileCfg cfg = cfg_in;
         int lx = pick_lx(g, VW, cfg);
+        if (MODE == MODE_AC2 && sizeof(TS) == 4 && sizeof(TC) == 4 && cfg.lx == 0 && cfg.by == 0 &&
+            wide_by_default(g) && wide_tiles_fit(g))
+            lx = 128;
         return dispatch_R(R, [&](auto rc_) -> int {
             constexpr int RR = decltype(rc_)::value;
             if (lx == 128) {
@@ -1029,23 +1062,6 @@ inline int run_tiled_shape(const hsx_grid *g, void *out, const void *prn_in, con
             return launch_tiled_by<TS, TC, RR, VW, 64, MODE, FOLD>(g, out, prn_in, u, coef, s, cfg);
         });
     });
-}
-
-// whether the 128 x 8 tiles exist for g (FP32, the ring's LDS fits, no spill):
-// elsewhere a request for them runs the library's choice
-inline bool wide_tiles_fit(const hsx_grid *g)
-{
-    if (g->storage != HSX_FP32 || g->compute != HSX_FP32 || g->dim != 3) return false;
-    return dispatch_R(g->fd_order / 2, [&](auto rc_) -> int {
-               constexpr int RR = decltype(rc_)::value;
-               using K = Tiled<float, float, RR, 4, 128, 8, MODE_AC2, true>;
-               if constexpr (K::RING && K::LDS <= TILE_LDS_MAX) {
-                   static const bool spills = tiled_spills(k_tiled<float, float, RR, 4, 128, 8, MODE_AC2, true>);
-                   return spills ? 0 : 1;
-               } else {
-                   return 0;
-               }
-           }) == 1;
 }
 
 // the forced shape (hsx_set_tile) if any, else the tuned one, else the defaults

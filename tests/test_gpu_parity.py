@@ -401,6 +401,9 @@ TILED_CASES = [
     (16, "f32", 260, 40, 36, 128, 0, 8, 0),    # R = 8: ring too large, library choice
     (8, "f64", 300, 20, 22, 128, 0, 8, 0),     # FP64: library choice
     (8, "f32", 560, 21, 40, 128, 0, 8, 4),     # 128 lanes without the ring: library choice
+    (8, "f32", 480, 21, 19, 0, 0, 0, 0),       # library choice: 128 x 8 (one 512-point tile per row)
+    (8, "f32", 960, 13, 17, 0, 0, 0, 0),       # library choice: 128 x 8 (two tiles, 94 % of the width)
+    (8, "f32", 800, 13, 17, 0, 0, 0, 0),       # library choice: 64 x 16 (128 lanes would fill 78 %)
 ]
 
 
