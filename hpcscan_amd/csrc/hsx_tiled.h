@@ -975,21 +975,9 @@ inline int tiled_fits(const hsx_grid *g, int vw_elems, size_t elem)
 bool tuned_shape(const hsx_grid *g, int mode, TileCfg *out);
 
 // whether the 128 x 8 tiles exist for g (FP32, the ring's LDS fits, spills
-// within TILED_SPILL_OK): elsewhere a request for them runs the library's choice
-inline bool wide_tiles_fit(const hsx_grid *g)
-{
-    if (g->storage != HSX_FP32 || g->compute != HSX_FP32 || g->dim != 3) return false;
-    return dispatch_R(g->fd_order / 2, [&](auto rc_) -> int {
-               constexpr int RR = decltype(rc_)::value;
-               using K = Tiled<float, float, RR, 4, 128, 8, MODE_AC2, true>;
-               if constexpr (K::RING && K::LDS <= TILE_LDS_MAX) {
-                   static const bool spills = tiled_spills(k_tiled<float, float, RR, 4, 128, 8, MODE_AC2, true>);
-                   return spills ? 0 : 1;
-               } else {
-                   return 0;
-               }
-           }) == 1;
-}
+// within TILED_SPILL_OK): elsewhere a request for them runs the library's
+// choice (hsx_tiled_ac2.hip, which holds their instances)
+bool wide_tiles_fit(const hsx_grid *g);
 
 // The fused step's own choice of 128 x 8 tiles (with the ring) over 64 x 16:
 // where both span the same width and either one 512-point tile covers the row

@@ -36,6 +36,22 @@ int set_tile(int lx, int by, int chunk, int flags)
 
 thread_local FoldReq t_fold = {0, {-1, -1, -1, -1, -1, -1}};
 
+bool wide_tiles_fit(const hsx_grid *g)
+{
+    if (g->storage != HSX_FP32 || g->compute != HSX_FP32 || g->dim != 3) return false;
+    return dispatch_R(g->fd_order / 2, [&](auto rc_) -> int {
+               constexpr int RR = decltype(rc_)::value;
+               using K = Tiled<float, float, RR, 4, 128, 8, MODE_AC2, true>;
+               if constexpr (K::RING && K::LDS <= TILE_LDS_MAX) {
+                   static const bool spills = tiled_spills(k_tiled<float, float, RR, 4, 128, 8, MODE_AC2, true>);
+                   return spills ? 0 : 1;
+               } else {
+                   return 0;
+               }
+           }) == 1;
+}
+
+
 int tiled_ac2_step(const hsx_grid *g, void *out, const void *prn, const void *prc, const void *coef, hipStream_t s)
 {
     if (contract_mode()) return tiled_ac2_step_fma(g, out, prn, prc, coef, s);
