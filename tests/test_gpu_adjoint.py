@@ -102,10 +102,11 @@ def test_receivers_on_source_points(oracle, capi, nsrc, fp64):
     _same(grad.download(np.float64), gref)
 
 
-@pytest.mark.parametrize("fmt,tol", [("fp16", 1e-3), ("bf16", 1e-2)])
+@pytest.mark.parametrize("fmt,tol", [("fp16", 1e-3), ("bf16", 1e-3)])
 def test_low_precision_checkpoints_against_fp64_oracle(oracle, capi, fmt, tol):
     """FP32 stencil, 16-bit scaled checkpoints, FP64 gradient accumulation: the
-    error against the FP64 oracle gradient"""
+    error against the FP64 oracle gradient within 1e-3 relative L2 for both
+    16-bit formats"""
     sh64 = make_shot(oracle, (40, 36, 32), 8, nt=120, fp64=True, seed=21)
     sh = make_shot(oracle, (40, 36, 32), 8, nt=120, fp64=False, seed=21)
     g64, _ = oracle.gradient(sh64["o"], sh64["coef"], sh64["iv"], sh64["dt"], sh64["nt"], sh64["src"],
@@ -287,7 +288,7 @@ def test_16bit_checkpoints_odd_point_count(oracle, capi, fmt):
     err = float(np.linalg.norm(x - g64) / np.linalg.norm(g64))
     print(f"{fmt} checkpoints every 7, odd npoint {sh['p'].npoint}: rel L2 vs FP64 oracle {err:.3e}", flush=True)
     assert np.all(x[~sh["grad_mask"]] == 0)
-    assert err < {"fp16": 1e-3, "bf16": 1e-2}[fmt]
+    assert err < 1e-3
 
 
 def test_plan_and_given_residual(oracle, capi):

@@ -19,7 +19,7 @@
   device FP64 forward, and the 16-bit checkpoint gradients against the FP64
   gradient.
 * One 512^3 order-16 x 2000-step shot: 16-bit checkpoint gradients against the
-  FP64 gradient (relative L2 <= 1e-3 for FP16; the BF16 figure is printed).
+  FP64 gradient (relative L2 <= 1e-3 for FP16 and BF16; the figures are printed).
 """
 import math
 import os
@@ -400,7 +400,7 @@ def test_adjoint_200_cube_1000_steps(capi):
     print(f"200^3 x 1000: J {J64:.6e}; <grad, dm> {adj:.9e}, central difference {fd:.9e}", flush=True)
     assert adj == pytest.approx(fd, rel=1e-5)
     nrm = np.linalg.norm(g64)
-    for name, fmt, tol in (("fp16", capi.HSX_CKPT_FP16, 1e-3), ("bf16", capi.HSX_CKPT_BF16, 1e-2)):
+    for name, fmt, tol in (("fp16", capi.HSX_CKPT_FP16, 1e-3), ("bf16", capi.HSX_CKPT_BF16, 1e-3)):
         g, J = _grad(capi, A, False, 20, fmt, A["dobs32"])
         err = float(np.linalg.norm(g - g64) / nrm)
         print(f"  {name} checkpoints: relative L2 vs FP64 {err:.3e}, misfit {J:.6e}", flush=True)
@@ -415,7 +415,7 @@ def test_adjoint_512_cube_order16_2000_steps(capi):
     g64, _ = _grad(capi, A, True, 63, capi.HSX_CKPT_FP32, A["d64"])
     nrm = np.linalg.norm(g64)
     assert nrm > 0
-    for name, fmt, tol in (("fp16", capi.HSX_CKPT_FP16, 1e-3), ("bf16", capi.HSX_CKPT_BF16, 2e-2)):
+    for name, fmt, tol in (("fp16", capi.HSX_CKPT_FP16, 1e-3), ("bf16", capi.HSX_CKPT_BF16, 1e-3)):
         g, _ = _grad(capi, A, False, 20, fmt, A["dobs32"])
         err = float(np.linalg.norm(g - g64) / nrm)
         print(f"512^3 O16 x 2000, {name} checkpoints every 20: relative L2 vs FP64 {err:.3e}", flush=True)
