@@ -38,21 +38,72 @@ __device__ inline void for_box(const Box &b, int64_t n1, int64_t n12, F &&f)
     }
 }
 
+// Visit box b row by row in 16-byte groups of VW points (flat indices aligned
+// to VW, so the buffer must be 16-byte aligned): group(ii, i1, i2, i3) for each
+// whole group inside the row's [s1, e1] span (points ii .. ii + VW - 1, i1 the
+// first one's n1 index), point(ii, i1, i2, i3) for the unaligned head and tail
+// points of the row.  One wave per row, as for_box.
+template <int VW, typename FP, typename FG>
+__device__ inline void for_box_groups(const Box &b, int64_t n1, int64_t n12, FP &&point, FG &&group)
+{
+    const int64_t N2 = b.n2(), rows = N2 * b.n3();
+    const int lane = threadIdx.x;
+    for (int64_t row = (int64_t)blockIdx.x * ROWS_PER_BLOCK + threadIdx.y; row < rows;
+         row += (int64_t)gridDim.x * ROWS_PER_BLOCK) {
+        const int64_t i2 = b.s2 + row % N2, i3 = b.s3 + row / N2;
+        const int64_t base = i2 * n1 + i3 * n12;
+        const int64_t a0 = base + b.s1, a1 = base + b.e1 + 1;       // [a0, a1) flat
+        const int64_t g0 = (a0 + VW - 1) / VW, g1 = a1 / VW;          // whole groups
+        if (g0 >= g1) {
+            for (int64_t i = a0 + lane; i < a1; i += ROW_THREADS) point(i, i - base, i2, i3);
+            continue;
+        }
+        const int64_t h = g0 * VW - a0, t = a1 - g1 * VW;
+        if (lane < h) point(a0 + lane, a0 + lane - base, i2, i3);
+        if (lane < t) point(g1 * VW + lane, g1 * VW + lane - base, i2, i3);
+        for (int64_t gi = g0 + lane; gi < g1; gi += ROW_THREADS) group(gi * VW, gi * VW - base, i2, i3);
+    }
+}
+
 // ---- fills (src/grid.cpp:1650-1826) -----------------------------------------
-template <typename TS>
+// VEC: 16-byte stores (u 16-byte aligned)
+template <typename TS, bool VEC>
 __global__ void __launch_bounds__(256) k_fill_const(TS *u, Box b, int64_t n1, int64_t n12, TS val)
 {
-    for_box(b, n1, n12, [&](int64_t ii, int64_t, int64_t, int64_t) { u[ii] = val; });
+    if constexpr (!VEC) {
+        for_box(b, n1, n12, [&](int64_t ii, int64_t, int64_t, int64_t) { u[ii] = val; });
+    } else {
+        constexpr int VW = 16 / sizeof(TS);
+        using NV = TS __attribute__((ext_vector_type(VW)));
+        const NV vv = (NV)val;
+        for_box_groups<VW>(
+            b, n1, n12, [&](int64_t ii, int64_t, int64_t, int64_t) { u[ii] = val; },
+            [&](int64_t ii, int64_t, int64_t, int64_t) { *reinterpret_cast<NV *>(u + ii) = vv; });
+    }
 }
 
 // amp * v1 * v2 * v3 left to right in double, then rounded to the storage type
-template <typename TS>
+template <typename TS, bool VEC>
 __global__ void __launch_bounds__(256) k_fill_sep(TS *u, Box b, int64_t n1, int64_t n12, const double *t1,
                                                   const double *t2, const double *t3, double amp)
 {
-    for_box(b, n1, n12, [&](int64_t ii, int64_t i1, int64_t i2, int64_t i3) {
-        u[ii] = (TS)(amp * t1[i1 - b.s1] * t2[i2 - b.s2] * t3[i3 - b.s3]);
-    });
+    auto one = [&](int64_t i1, int64_t i2, int64_t i3) -> TS {
+        return (TS)(amp * t1[i1 - b.s1] * t2[i2 - b.s2] * t3[i3 - b.s3]);
+    };
+    if constexpr (!VEC) {
+        for_box(b, n1, n12, [&](int64_t ii, int64_t i1, int64_t i2, int64_t i3) { u[ii] = one(i1, i2, i3); });
+    } else {
+        constexpr int VW = 16 / sizeof(TS);
+        using NV = TS __attribute__((ext_vector_type(VW)));
+        for_box_groups<VW>(
+            b, n1, n12, [&](int64_t ii, int64_t i1, int64_t i2, int64_t i3) { u[ii] = one(i1, i2, i3); },
+            [&](int64_t ii, int64_t i1, int64_t i2, int64_t i3) {
+                NV r;
+#pragma unroll
+                for (int e = 0; e < VW; e++) r[e] = one(i1 + e, i2, i3);
+                *reinterpret_cast<NV *>(u + ii) = r;
+            });
+    }
 }
 
 // ---- array operations (src/grid.cpp:1830-1920) --------------------------------
@@ -637,31 +688,84 @@ __device__ inline void block_reduce(double &a, double &b)
     }
 }
 
+// one point of a reduction: the sums in double, min / max in the storage type
+// as the reference's Myfloat accumulator (src/grid.cpp:2006-2370)
 template <typename TS, int OP>
+__device__ __forceinline__ void red_point(TS x, TS y, double &a, double &c, TS &mn, TS &mx, float maxerr)
+{
+    if (OP == RED_SUM_ABS) {
+        a += fabs((double)x);
+    } else if (OP == RED_SUM_ABS_DIFF) {
+        a += fabs((double)x - (double)y);
+    } else if (OP == RED_L1) {
+        a += fabs((double)x - (double)y);
+        c += fabs((double)y);
+    } else if (OP == RED_MAX_ERR) {
+        const double d = fabs((double)x - (double)y);
+        const double e = fabsf((float)y) < maxerr ? d : d / fabs((double)y);
+        if (e > a) a = e;
+    } else {
+        if (x < mn) mn = x;
+        if (x > mx) mx = x;
+    }
+}
+
+// VEC: 16-byte loads over the aligned groups of each row (u and v 16-byte
+// aligned), four loads in flight per operand, the unaligned head and tail of
+// the row point by point; one wave per row either way.  1000^3 FP32 inner
+// box: sum |u| 0.68 ms (1.86 ms point by point), sum |u - v| 1.40 ms (2.26 ms).
+template <typename TS, int OP, bool VEC>
 __global__ void __launch_bounds__(256) k_reduce(const TS *u, const TS *v, Box b, int64_t n1, int64_t n12,
                                                 float maxerr, double *part)
 {
     double a = RedInit<OP>::a(), c = RedInit<OP>::b();
-    // min/max run in the storage type, as the reference's Myfloat accumulator does
     TS mn = (TS)FLT_MAX, mx = (TS)-FLT_MAX;
-    for_box(b, n1, n12, [&](int64_t ii, int64_t, int64_t, int64_t) {
-        if (OP == RED_SUM_ABS) {
-            a += fabs((double)u[ii]);
-        } else if (OP == RED_SUM_ABS_DIFF) {
-            a += fabs((double)u[ii] - (double)v[ii]);
-        } else if (OP == RED_L1) {
-            a += fabs((double)u[ii] - (double)v[ii]);
-            c += fabs((double)v[ii]);
-        } else if (OP == RED_MAX_ERR) {
-            const double d = fabs((double)u[ii] - (double)v[ii]);
-            const double e = fabsf((float)v[ii]) < maxerr ? d : d / fabs((double)v[ii]);
-            if (e > a) a = e;
-        } else {
-            const TS x = u[ii];
-            if (x < mn) mn = x;
-            if (x > mx) mx = x;
+    constexpr bool TWO = OP != RED_SUM_ABS && OP != RED_MIN_MAX;
+    if constexpr (!VEC) {
+        for_box(b, n1, n12, [&](int64_t ii, int64_t, int64_t, int64_t) {
+            red_point<TS, OP>(u[ii], TWO ? v[ii] : (TS)0, a, c, mn, mx, maxerr);
+        });
+    } else {
+        constexpr int VW = 16 / sizeof(TS), UN = 4;                    // 4 loads in flight per operand
+        using NV = TS __attribute__((ext_vector_type(VW)));
+        const int64_t N2 = b.n2(), rows = N2 * b.n3();
+        const int lane = threadIdx.x;
+        for (int64_t row = (int64_t)blockIdx.x * ROWS_PER_BLOCK + threadIdx.y; row < rows;
+             row += (int64_t)gridDim.x * ROWS_PER_BLOCK) {
+            const int64_t base = (b.s2 + row % N2) * n1 + (b.s3 + row / N2) * n12;
+            const int64_t a0 = base + b.s1, a1 = base + b.e1 + 1;       // [a0, a1) flat
+            const int64_t g0 = (a0 + VW - 1) / VW, g1 = a1 / VW;          // whole groups
+            if (g0 >= g1) {                                               // short row
+                for (int64_t i = a0 + lane; i < a1; i += ROW_THREADS)
+                    red_point<TS, OP>(u[i], TWO ? v[i] : (TS)0, a, c, mn, mx, maxerr);
+                continue;
+            }
+            const int64_t h = g0 * VW - a0, t = a1 - g1 * VW;             // head / tail points
+            if (lane < h) red_point<TS, OP>(u[a0 + lane], TWO ? v[a0 + lane] : (TS)0, a, c, mn, mx, maxerr);
+            if (lane < t) {
+                const int64_t i = g1 * VW + lane;
+                red_point<TS, OP>(u[i], TWO ? v[i] : (TS)0, a, c, mn, mx, maxerr);
+            }
+            for (int64_t gi = g0 + lane; gi < g1; gi += UN * ROW_THREADS) {
+                NV x[UN], y[UN];
+#pragma unroll
+                for (int k = 0; k < UN; k++) {
+                    const int64_t gk = gi + k * ROW_THREADS;
+                    if (gk < g1) {
+                        x[k] = *reinterpret_cast<const NV *>(u + gk * VW);
+                        if constexpr (TWO) y[k] = *reinterpret_cast<const NV *>(v + gk * VW);
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < UN; k++) {
+                    if (gi + k * ROW_THREADS >= g1) break;
+#pragma unroll
+                    for (int e = 0; e < VW; e++)
+                        red_point<TS, OP>(x[k][e], TWO ? y[k][e] : (TS)0, a, c, mn, mx, maxerr);
+                }
+            }
         }
-    });
+    }
     if (OP == RED_MIN_MAX) {
         a = (double)mn;
         c = (double)mx;
@@ -671,6 +775,12 @@ __global__ void __launch_bounds__(256) k_reduce(const TS *u, const TS *v, Box b,
         part[blockIdx.x] = a;
         part[gridDim.x + blockIdx.x] = c;
     }
+}
+
+// the 16-byte path where both operands are 16-byte aligned
+static bool red_vec(const void *u, const void *v)
+{
+    return ((uintptr_t)u % 16) == 0 && ((uintptr_t)v % 16) == 0;
 }
 
 template <int OP>
@@ -746,8 +856,12 @@ static int run_reduce(const hsx_grid *g, const void *u, const void *v, const int
     hipStream_t s = stream_of(stream);
     rc = dispatch_types(g, [&](auto ts, auto) -> int {
         using TS = decltype(ts);
-        hipLaunchKernelGGL((k_reduce<TS, OP>), dim3(nb), dim3(ROW_THREADS, ROWS_PER_BLOCK), 0, s, (const TS *)u,
-                           (const TS *)v, b, g->n1, g->n1 * g->n2, maxerr, part);
+        if (red_vec(u, v))
+            hipLaunchKernelGGL((k_reduce<TS, OP, true>), dim3(nb), dim3(ROW_THREADS, ROWS_PER_BLOCK), 0, s,
+                               (const TS *)u, (const TS *)v, b, g->n1, g->n1 * g->n2, maxerr, part);
+        else
+            hipLaunchKernelGGL((k_reduce<TS, OP, false>), dim3(nb), dim3(ROW_THREADS, ROWS_PER_BLOCK), 0, s,
+                               (const TS *)u, (const TS *)v, b, g->n1, g->n1 * g->n2, maxerr, part);
         HSX_LAUNCHED("k_reduce");
         return HSX_OK;
     });
@@ -783,8 +897,12 @@ int hsx_fill_const(const hsx_grid *g, void *u, const int64_t range[6], double va
     hipStream_t s = stream_of(stream);
     return dispatch_types(g, [&](auto ts, auto) -> int {
         using TS = decltype(ts);
-        hipLaunchKernelGGL(k_fill_const<TS>, row_grid(b), dim3(ROW_THREADS, ROWS_PER_BLOCK), 0, s, (TS *)u, b,
-                           g->n1, g->n1 * g->n2, (TS)val);
+        if (((uintptr_t)u % 16) == 0)
+            hipLaunchKernelGGL((k_fill_const<TS, true>), row_grid(b), dim3(ROW_THREADS, ROWS_PER_BLOCK), 0, s,
+                               (TS *)u, b, g->n1, g->n1 * g->n2, (TS)val);
+        else
+            hipLaunchKernelGGL((k_fill_const<TS, false>), row_grid(b), dim3(ROW_THREADS, ROWS_PER_BLOCK), 0, s,
+                               (TS *)u, b, g->n1, g->n1 * g->n2, (TS)val);
         HSX_LAUNCHED("k_fill_const");
         return HSX_OK;
     });
@@ -808,8 +926,12 @@ int hsx_fill_separable(const hsx_grid *g, void *u, const int64_t range[6], const
     HSX_HIP(hipMemcpyAsync(tab + l1 + l2, t3, sizeof(double) * l3, hipMemcpyHostToDevice, s));
     rc = dispatch_types(g, [&](auto ts, auto) -> int {
         using TS = decltype(ts);
-        hipLaunchKernelGGL(k_fill_sep<TS>, row_grid(b), dim3(ROW_THREADS, ROWS_PER_BLOCK), 0, s, (TS *)u, b,
-                           g->n1, g->n1 * g->n2, tab, tab + l1, tab + l1 + l2, amp);
+        if (((uintptr_t)u % 16) == 0)
+            hipLaunchKernelGGL((k_fill_sep<TS, true>), row_grid(b), dim3(ROW_THREADS, ROWS_PER_BLOCK), 0, s,
+                               (TS *)u, b, g->n1, g->n1 * g->n2, tab, tab + l1, tab + l1 + l2, amp);
+        else
+            hipLaunchKernelGGL((k_fill_sep<TS, false>), row_grid(b), dim3(ROW_THREADS, ROWS_PER_BLOCK), 0, s,
+                               (TS *)u, b, g->n1, g->n1 * g->n2, tab, tab + l1, tab + l1 + l2, amp);
         HSX_LAUNCHED("k_fill_sep");
         return HSX_OK;
     });
@@ -986,8 +1108,12 @@ int hsx_l1_sums_noalloc(const hsx_grid *g, const void *u, const void *v, const i
     // nothing in scratch has to start at zero
     rc = dispatch_types(g, [&](auto ts, auto) -> int {
         using TS = decltype(ts);
-        hipLaunchKernelGGL((k_reduce<TS, RED_L1>), dim3(nb), dim3(ROW_THREADS, ROWS_PER_BLOCK), 0, s, (const TS *)u,
-                           (const TS *)v, b, g->n1, g->n1 * g->n2, 0.f, part);
+        if (red_vec(u, v))
+            hipLaunchKernelGGL((k_reduce<TS, RED_L1, true>), dim3(nb), dim3(ROW_THREADS, ROWS_PER_BLOCK), 0, s,
+                               (const TS *)u, (const TS *)v, b, g->n1, g->n1 * g->n2, 0.f, part);
+        else
+            hipLaunchKernelGGL((k_reduce<TS, RED_L1, false>), dim3(nb), dim3(ROW_THREADS, ROWS_PER_BLOCK), 0, s,
+                               (const TS *)u, (const TS *)v, b, g->n1, g->n1 * g->n2, 0.f, part);
         HSX_LAUNCHED("k_reduce");
         return HSX_OK;
     });

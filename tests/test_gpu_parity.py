@@ -230,6 +230,42 @@ def test_fills_bit_exact(oracle, capi, prec):
 
 
 @pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("n", [(3, 5, 4), (7, 9, 6), (1100, 5, 4)])
+@pytest.mark.parametrize("shift", [0, 1])
+def test_fills_vector_and_point_paths(oracle, capi, prec, n, shift):
+    """16-byte stores with point-by-point row heads and tails when the grid is
+    16-byte aligned (shift 0), point by point otherwise (shift 1); the element
+    before a shifted grid stays untouched"""
+    dtype, _ = PRECS[prec]
+    it = np.dtype(dtype).itemsize
+    p, o = _layouts(oracle, 3, n, 4, dtype)
+    g = capi.hsx_grid(p)
+    base = np.zeros(p.npoint + shift, dtype)
+    base[:shift] = dtype(5.5)
+    d = capi.DeviceArray.from_numpy(base)
+    pu = d.ptr.value + shift * it
+
+    def got():
+        out = d.download(dtype)
+        assert (out[:shift] == dtype(5.5)).all()
+        return out[shift:]
+
+    for ptype in (oracle.INNER_POINTS, oracle.ALL_POINTS, oracle.I2HALO1, oracle.MIDDLE_POINT):
+        ref = base[shift:].copy()
+        oracle.fill_const(o, ref, ptype, 0.7)
+        d.upload(base)
+        capi.fill_const(g, pu, oracle.index_range(o, ptype), dtype(0.7))
+        _same(got(), ref)
+    orig = [oracle.unit_grid(n[a], o.ax[a].inner_start, fp64=dtype == np.float64)[1] for a in range(3)]
+    for ptype in (oracle.INNER_POINTS, oracle.ALL_POINTS):
+        ref = base[shift:].copy()
+        tabs = oracle.fill_sep(o, ref, ptype, (1, 2, 1), (2.1, 0.7, 3.3), -1.3, orig, p.d, 3)
+        d.upload(base)
+        capi.fill_separable(g, pu, oracle.index_range(o, ptype), *tabs, -1.3)
+        _same(got(), ref)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
 def test_reductions(oracle, capi, prec):
     dtype, _ = PRECS[prec]
     p, o = _layouts(oracle, 3, (40, 33, 21), 8, dtype)
@@ -252,6 +288,37 @@ def test_reductions(oracle, capi, prec):
         mn, mx = capi.min_max(g, du, r)
         assert mn == oracle.reduce("min", o, u, ptype=ptype)
         assert mx == oracle.reduce("max", o, u, ptype=ptype)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("n", [(3, 5, 4), (7, 9, 6), (1100, 5, 4)])
+@pytest.mark.parametrize("shift", [0, 1])
+def test_reductions_vector_and_point_paths(oracle, capi, prec, n, shift):
+    """16-byte loads with a point-by-point head and tail on each row when both
+    buffers are 16-byte aligned (shift 0), point by point otherwise (shift 1):
+    rows shorter than one vector group, rows of a few groups, and rows of
+    several 4 x 64-group passes with a partial last pass"""
+    dtype, _ = PRECS[prec]
+    it = np.dtype(dtype).itemsize
+    p, o = _layouts(oracle, 3, n, 4, dtype)
+    rng = np.random.default_rng(23)
+    u = _rand(rng, p.npoint, dtype)
+    v = _rand(rng, p.npoint, dtype)
+    v[::5] = dtype(1e-7)
+    g = capi.hsx_grid(p)
+    pad = np.zeros(shift, dtype)
+    du = capi.DeviceArray.from_numpy(np.concatenate([pad, u]))
+    dv = capi.DeviceArray.from_numpy(np.concatenate([pad, v]))
+    pu, pv = du.ptr.value + shift * it, dv.ptr.value + shift * it
+    maxerrf = 1e-11 if dtype == np.float64 else 1e-5
+    for ptype in (oracle.INNER_POINTS, oracle.ALL_POINTS):
+        r = oracle.index_range(o, ptype)
+        assert abs(capi.sum_abs(g, pu, r) / oracle.reduce("sum_abs", o, u, ptype=ptype) - 1) < 1e-12
+        assert abs(capi.sum_abs_diff(g, pu, pv, r) / oracle.reduce("sum_abs_diff", o, u, v, ptype=ptype) - 1) < 1e-12
+        s1, s2 = capi.l1_sums(g, pu, pv, r)
+        assert abs(s1 / s2 / oracle.reduce("l1_err", o, u, v, ptype=ptype) - 1) < 1e-12
+        assert capi.max_err(g, pu, pv, r, maxerrf) == oracle.reduce("max_err", o, u, v, ptype=ptype)
+        assert capi.min_max(g, pu, r) == (oracle.reduce("min", o, u, ptype=ptype), oracle.reduce("max", o, u, ptype=ptype))
 
 
 @pytest.mark.parametrize("fp64", [False, True])
