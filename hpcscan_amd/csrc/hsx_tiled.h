@@ -496,7 +496,10 @@ __device__ __forceinline__ void tiled_body_tz(const TiledArgs<TC> &a, int tile, 
                 }
             }
 
-            if (K::NEED_LDS) __syncthreads();
+            // n3-only kernels (no LDS) meet here too: it keeps the
+            // workgroup's waves on one plane (FD_D2 along n3, 1000^3 FP32:
+            // order 8 1.85 ms against 1.97 ms without, order 16 1.68 / 1.85)
+            __syncthreads();
 
             if (any_inner) {
                 // n1 neighbours: the row segment [tx*VW, tx*VW + VW + 2 HL) of the LDS row
@@ -831,6 +834,24 @@ inline bool tiled_spills(KERN kern)
     return attr.localSizeBytes > TILED_SPILL_OK;
 }
 
+// n3-only kernels (D2 and D1 along n3: register queue, no LDS): the chunk
+// length that gives about 1024 threads per CU in one round of workgroups
+// (tiles x chunks <= CUs x 1024 / threads, as many chunks as fit), or 0
+// when the tiles alone fill less than 3/4 of that round (then the general
+// rule).  FD_D2 along n3, nontemporal streams, against the general rule:
+// FP32 order 8 (64 x 16 tiles) 1000^3 1.71 / 1.84 ms, 500^3 0.201 / 0.233
+// ms, 300^3 0.040 / 0.050 ms; FP64 order 8 (64 x 8) 1000^3 3.82 / 3.87 ms,
+// 700^3 1.31 / 1.38 ms, 500^3 0.419 / 0.488 ms, 300^3 0.069 / 0.108 ms;
+// FP32 order 16 1000^3 and 500^3 unchanged, 300^3 0.056 / 0.048 ms
+inline int64_t chunk_stream(int64_t N3, int R, int64_t tiles, int threads)
+{
+    const int64_t want = (int64_t)device_cus() * 1024 / threads;
+    const int64_t k = std::max<int64_t>(1, want / tiles);
+    if (k == 1 && 4 * tiles < 3 * want) return 0;
+    const int64_t c = (N3 + k - 1) / k;
+    return std::min<int64_t>(std::max<int64_t>(c, 4 * R), N3);
+}
+
 // kernel arguments of one grid; slots = resident workgroups available to it
 // (chunk > 0: n3 planes per workgroup, at least 4R, unless hsx_set_tile forces one)
 template <typename TS, typename TC, int R, int VW, int LX, int BY, int MODE, bool RING = false>
@@ -858,14 +879,17 @@ inline int fill_tiled_args(const hsx_grid *g, void *out, const void *prn_in, con
     a.nty = (int)((a.e2 - a.s2 + 1 + BY - 1) / BY);
     const int64_t N3 = a.e3 - a.s3 + 1;
     const int64_t tiles = (int64_t)a.ntx * a.nty;
+    constexpr bool STREAM = !Tiled<TS, TC, R, VW, LX, BY, MODE, RING>::NEED_LDS;
+    const int64_t cs = STREAM && cfg.chunk <= 0 && chunk <= 0 ? chunk_stream(N3, R, tiles, LX * BY) : 0;
     const int64_t len = cfg.chunk > 0 ? std::min<int64_t>(cfg.chunk, N3)
                         : chunk > 0           ? std::min<int64_t>(std::max<int64_t>(chunk, 4 * R), N3)
+                        : cs > 0              ? cs
                         : (BY == 16 || RING) ? chunk_one_round(N3, R, tiles, slots)
                                           : chunk_short(N3, R, tiles);
     a.chunk = (int)len;
     a.nchunk = (int)((N3 + len - 1) / len);
     const int streams = cfg.flags & (HSX_TILE_CACHED | HSX_TILE_NT);
-    a.nt = streams == HSX_TILE_NT || (streams == 0 && (BY == 16 || RING));
+    a.nt = streams == HSX_TILE_NT || (streams == 0 && (BY == 16 || RING || STREAM));
     a.fold = MODE == MODE_AC2 ? t_fold.flags : 0;   // the folded free surface: exact step only
     for (int k = 0; k < 6; k++) a.fb[k] = a.fold ? (int)t_fold.fb[k] : -1;
     if constexpr (MODE == MODE_AC2_IMG) {
@@ -1016,10 +1040,11 @@ inline int run_tiled_shape(const hsx_grid *g, void *out, const void *prn_in, con
                 // 128 lanes x 8 rows with the halo ring (rows of 2 KiB, two
                 // waves each): the exact FP32 fused step and Laplacian up to
                 // order 8, where the ring's LDS fits and spills stay within
-                // TILED_SPILL_OK; on request only (hsx_set_tile, the
-                // autotuner). 1000^3 order 8:
-                // 3.04-3.08 ms against 3.12-3.19 ms for 64 x 16; 500^3 0.363
-                // against 0.394 ms; 700^3 and 800^3 6 % and 2 % slower.
+                // TILED_SPILL_OK; the fused step's default where
+                // wide_by_default holds, otherwise on request (hsx_set_tile,
+                // the autotuner). 1000^3 order 8: 3.04-3.08 ms against
+                // 3.12-3.19 ms for 64 x 16; 500^3 0.363 against 0.394 ms;
+                // 700^3 and 800^3 6 % and 2 % slower.
                 using K = Tiled<TS, TC, RR, VW, 128, 8, MODE, true>;
                 if constexpr (sizeof(TS) == 4 && sizeof(TC) == 4 && !FOLD && (MODE == MODE_AC2 || MODE == MODE_LAP) &&
                               K::RING && K::LDS <= TILE_LDS_MAX) {
