@@ -928,12 +928,14 @@ inline int launch_tiled(const hsx_grid *g, void *out, const void *prn_in, const 
 }
 
 // 16-row instance to use at LX lanes: 1 with the halo ring, 2 without, 0 none.
-// 1024-thread tiles leave 128 VGPRs per lane, too few for FP64 values; by
-// default an instance is used only if it keeps its values in registers (no
-// per-lane scratch); rows forced to 16 are used as asked
+// 1024-thread tiles leave 128 VGPRs per lane, too few for FP64 values with an
+// n3 register queue (the kernels without one, D2 and D1 along n1 and n2, keep
+// few); by default an instance is used only if it keeps its values in
+// registers (no per-lane scratch); rows forced to 16 are used as asked
 template <typename TS, typename TC, int R, int VW, int LX, int MODE>
 struct TallFits {
-    static constexpr bool fits_regs = (sizeof(TS) == 4 && sizeof(TC) == 4) || LX * 16 <= 512;
+    static constexpr bool fits_regs = (sizeof(TS) == 4 && sizeof(TC) == 4) || LX * 16 <= 512 ||
+                                      !Tiled<TS, TC, R, VW, LX, 16, MODE, false>::NEED_Z;
     static constexpr bool ring = LX >= 32 && fits_regs && Tiled<TS, TC, R, VW, LX, 16, MODE, true>::RING &&
                                  Tiled<TS, TC, R, VW, LX, 16, MODE, true>::LDS <= TILE_LDS_MAX;
     static constexpr bool tall = LX >= 32 && fits_regs &&
@@ -1066,8 +1068,13 @@ inline int run_tiled_shape(const hsx_grid *g, void *out, const void *prn_in, con
             // 3.70 against 4.27 ms; FP32 order 16: 4.09 against 3.90 ms (and
             // 4.50 ms for 64 x 16 with 8-byte lanes, which fits in registers
             // but issues twice the loads and LDS reads per point)
+            // (the kernels without an n3 queue, D2 and D1 along n1 and n2,
+            // take 64 x 16 in FP64 where it keeps its values in registers,
+            // else 32 x 16: 1000^3 FP64 D2 along n2 3.41 ms at order 8 and
+            // 4.27 ms at order 12 against 4.05 and 4.70 ms for 64 x 8)
             constexpr bool fp64_vals = !(sizeof(TS) == 4 && sizeof(TC) == 4);
-            const bool narrow = lx == 64 && cfg.lx == 0 && !(fp64_vals && RR <= 6) &&
+            constexpr bool queue = Tiled<TS, TC, RR, VW, 64, 16, MODE, false>::NEED_Z;
+            const bool narrow = lx == 64 && cfg.lx == 0 && !(fp64_vals && RR <= 6 && queue) &&
                                 tall_variant<TS, TC, RR, VW, 64, MODE, FOLD>(cfg) == 0 &&
                                 tall_variant<TS, TC, RR, VW, 32, MODE, FOLD>(cfg) != 0;
             if (lx == 32 || narrow)
