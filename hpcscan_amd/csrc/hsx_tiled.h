@@ -1033,9 +1033,20 @@ inline int run_tiled_shape(const hsx_grid *g, void *out, const void *prn_in, con
             return fail(HSX_ERR_UNSUPPORTED, "buffers not 16-byte aligned");
         TileCfg cfg = cfg_in;
         int lx = pick_lx(g, VW, cfg);
-        if (MODE == MODE_AC2 && sizeof(TS) == 4 && sizeof(TC) == 4 && cfg.lx == 0 && cfg.by == 0 &&
-            wide_by_default(g) && wide_tiles_fit(g))
+        // FP32 defaults measured per size (HIP events, kernel alone): the fused
+        // step and the order-4 Laplacian take 128 x 8 tiles where
+        // wide_by_default holds (Laplacian order 4: 1000^3 1.54 / 1.61 ms,
+        // 500^3 0.189 / 0.196, 400^3 0.119 / 0.122; 700^3, not taken, 0.671 /
+        // 0.587); the order-8 Laplacian drops the halo ring, whose halo rows
+        // still meet in L2 with two streams instead of four (1000^3 1.84 /
+        // 1.88 ms, 800^3 1.11 / 1.24, 700^3 0.70 / 0.79, 500^3 0.237 / 0.258,
+        // 400^3 0.131 / 0.141)
+        constexpr bool fp32_vals = sizeof(TS) == 4 && sizeof(TC) == 4;
+        const bool dflt = cfg.lx == 0 && cfg.by == 0;
+        if (fp32_vals && dflt && wide_by_default(g) &&
+            (MODE == MODE_AC2 ? wide_tiles_fit(g) : MODE == MODE_LAP && R == 2))
             lx = 128;
+        if (fp32_vals && dflt && cfg.flags == 0 && MODE == MODE_LAP && R == 4) cfg.flags = HSX_TILE_NO_RING;
         return dispatch_R(R, [&](auto rc_) -> int {
             constexpr int RR = decltype(rc_)::value;
             if (lx == 128) {
