@@ -1047,6 +1047,19 @@ inline int run_tiled_shape(const hsx_grid *g, void *out, const void *prn_in, con
             (MODE == MODE_AC2 ? wide_tiles_fit(g) : MODE == MODE_LAP && R == 2))
             lx = 128;
         if (fp32_vals && dflt && cfg.flags == 0 && MODE == MODE_LAP && R == 4) cfg.flags = HSX_TILE_NO_RING;
+        // FP64 Laplacian: order 12 on 32 x 16 tiles (1000^3 4.54 / 4.71 ms,
+        // 800^3 2.45 / 2.71, 700^3 1.65 / 1.93, 500^3 0.535 / 0.622, 400^3
+        // 0.363 / 0.425 against 64 x 8), order 16 on 16 x 8 (1000^3 5.55 /
+        // 6.49 ms, 800^3 3.04 / 3.47, 700^3 2.08 / 2.36, 500^3 0.802 / 0.864,
+        // 400^3 0.426 / 0.486 against 32 x 16)
+        if (sizeof(TS) == 8 && dflt && MODE == MODE_LAP && lx == 64) {
+            if (R == 6) {
+                lx = 32;
+                cfg.by = 16;
+            } else if (R == 8) {
+                lx = 16;
+            }
+        }
         return dispatch_R(R, [&](auto rc_) -> int {
             constexpr int RR = decltype(rc_)::value;
             if (lx == 128) {
